@@ -1,0 +1,126 @@
+/*
+ * rfd.h - C ABI of the MI355X (gfx950) continuous Frechet distance library.
+ *
+ * Plain C types only: pointers, sizes, scalars.  Curves are AoS FP64 arrays,
+ * point i at [i*dims + k], dims in {2, 3}.  Every entry point returns an
+ * rfd_status (0 = success); none throws.  One rfd_ctx per host thread; calls
+ * on one context are serialised on that context's HIP stream.
+ *
+ * Reference interfaces each entry point replaces (RobotFD):
+ *   src/frechet.cpp, include/frechet.hpp, src/compute_distance_wrapper.cpp,
+ *   include/intersection_algorithm_in_parallel.hpp (class Cuda_intersection).
+ */
+#ifndef RFD_H
+#define RFD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    RFD_OK = 0,
+    RFD_ERR_ARG = 1,        /* bad argument (null pointer, dims not 2/3, empty curve) */
+    RFD_ERR_HIP = 2,        /* a HIP runtime call failed */
+    RFD_ERR_NO_DEVICE = 3,  /* no GPU visible */
+    RFD_ERR_STATE = 4,      /* inconsistent result from a sweep (lb > ub, missing result) */
+    RFD_ERR_NOMEM = 5       /* device allocation failed */
+} rfd_status;
+
+/* Stop rule of the bisection (frechet.cpp:41). */
+typedef enum {
+    RFD_STOP_ABS = 0,  /* while (ub - lb > eps): the reference, eps = FLT_EPSILON */
+    RFD_STOP_REL = 1   /* while (ub - lb > eps * ub) */
+} rfd_stop;
+
+/* Arithmetic of the sweeps. */
+typedef enum {
+    RFD_PREC_FP64 = 0,   /* every sweep in FP64 */
+    RFD_PREC_MIXED = 1   /* FP32 VE-FD and FP32 decisions while the bracket is wide,
+                            FP64 decisions for the end game (certified bracket) */
+} rfd_precision;
+
+typedef struct {
+    int32_t stop_mode;     /* rfd_stop */
+    int32_t precision;     /* rfd_precision */
+    double eps;            /* stop tolerance */
+    int32_t vefd_gate;     /* 1: run VE-FD only when m*n > 10000 (.cu:1343); 0: always */
+    int32_t verbose;       /* 1: print the reference's progress lines (frechet.cpp:21-71) */
+    double mixed_switch;   /* MIXED: relative bracket width below which FP64 takes over */
+} rfd_opts;
+
+typedef struct {
+    double distance;       /* (lb + ub) / 2 at the end (frechet.cpp:63) */
+    double endpoint_lb;    /* frechet.cpp:30 */
+    double dfd;            /* upper bound from DFD<D> */
+    double vefd;           /* lower bound from VEFD<D> (endpoint_lb when gated off) */
+    double lb, ub;         /* final bracket */
+    double err_bound;      /* (ub - lb) / 2 / distance: relative error bound of `distance` */
+    int64_t iterations;    /* bisection steps (all precisions) */
+    int64_t iterations_fp32;
+    int64_t certify_steps; /* FP64 decisions spent certifying an FP32 bracket */
+    double ms_total;       /* wall time of the call on the device stream */
+} rfd_result;
+
+/* Per-kernel timing accumulated on a context's stream (HIP events). */
+typedef struct {
+    int64_t launches[8];   /* index: rfd_kernel_kind */
+    double ms[8];
+    double cells[8];       /* grid cells swept */
+} rfd_stats;
+
+typedef enum {
+    RFD_K_DFD = 0, RFD_K_VEFD64 = 1, RFD_K_DEC64 = 2, RFD_K_VEFD32 = 3, RFD_K_DEC32 = 4,
+    RFD_K_BATCH = 5, RFD_K_MLP = 6, RFD_K_OTHER = 7
+} rfd_kernel_kind;
+
+typedef struct rfd_ctx rfd_ctx;
+typedef struct rfd_session rfd_session;
+
+/* Library context: device, stream, timing.  No counterpart in the reference,
+ * which hard-codes device 0 and the default stream (.cu:32-37, 52). */
+rfd_status rfd_create(rfd_ctx **ctx, int device);
+rfd_status rfd_destroy(rfd_ctx *ctx);
+const char *rfd_status_string(rfd_status s);
+const char *rfd_last_error(rfd_ctx *ctx);
+void rfd_default_opts(rfd_opts *opts);
+rfd_status rfd_get_stats(rfd_ctx *ctx, rfd_stats *out);
+rfd_status rfd_reset_stats(rfd_ctx *ctx);
+
+/* Replaces Cuda_intersection::Cuda_intersection + data_move
+ * (.cu:23-142, 144-160): flattens nothing (input is already AoS), allocates
+ * the device buffers and copies both curves to the device. */
+rfd_status rfd_session_create(rfd_ctx *ctx, const double *P, int64_t m, const double *Q,
+                              int64_t n, int dims, rfd_session **out);
+/* Replaces Cuda_intersection::free_memory (.cu:162-175). */
+rfd_status rfd_session_destroy(rfd_session *s);
+
+/* Replaces Cuda_intersection::Bound_call_gpu(double *dfd, double *vefd)
+ * (.cu:1258-1395): *dfd <- DFD; *vefd is read as the endpoint bound and
+ * overwritten by VE-FD when run_vefd != 0 (the caller applies the m*n > 10000
+ * gate of .cu:1343).  fp32 != 0 evaluates VE-FD in FP32. */
+rfd_status rfd_session_bounds(rfd_session *s, double *dfd, double *vefd, int run_vefd,
+                              int fp32);
+
+/* Replaces Cuda_intersection::intersection_interval_call_gpu(double radius)
+ * (.cu:1398-1597): *reachable <- 1 if the free-space sweep at `radius` reaches
+ * (m-1, n-1), i.e. the terminal rightUB == 1.0 (.cu:1592).  *terminal_ub (may
+ * be NULL) receives that rightUB.  fp32 != 0 evaluates the sweep in FP32. */
+rfd_status rfd_session_decide(rfd_session *s, double radius, int fp32, int *reachable,
+                              double *terminal_ub);
+
+/* Replaces Frechet::Continuous::distance_cuda (frechet.cpp:18-75) on a
+ * session whose curves are already resident on the device. */
+rfd_status rfd_session_distance(rfd_session *s, const rfd_opts *opts, rfd_result *res);
+
+/* Whole call: create a session, run distance_cuda, destroy the session.
+ * compute_distance_parallel (compute_distance_wrapper.cpp:8-12) is
+ * rfd_distance with stop_mode = RFD_STOP_ABS, eps = FLT_EPSILON. */
+rfd_status rfd_distance(rfd_ctx *ctx, const double *P, int64_t m, const double *Q, int64_t n,
+                        int dims, const rfd_opts *opts, rfd_result *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RFD_H */

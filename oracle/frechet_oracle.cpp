@@ -1,0 +1,384 @@
+// CPU restatement of RobotFD's continuous Frechet distance path.
+//
+// TEST INFRASTRUCTURE ONLY: the checker for the GPU library.  Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+//
+// Built with -ffp-contract=off: every fused multiply-add below is written as an
+// explicit std::fma, and every other product/sum is rounded on its own, so the
+// GPU kernels (built the same way) can reproduce each value bit for bit.
+//
+// Canonical arithmetic.  The reference kernels are plain CUDA C compiled by
+// nvcc with its default --fmad=true, which fuses "acc += a*b" and
+// "t*x + y" into FMA.  This restatement makes those fusions explicit:
+//   * every "acc += a*b" accumulation (.cu:382-384, 392, 1180-1181, 1210-1211,
+//     619) is acc = fma(a, b, acc);
+//   * the projection "t*(b-a) + a" (.cu:390, 412) is fma(t, b - a, a);
+//   * the discriminant "B*B - 4.0*A*C" (.cu:1185, 1215) is
+//     fma(B, B, -((4.0*A)*C)).
+// Parity of these choices with nvcc's own contraction cannot be checked here
+// (no CUDA toolchain); the GPU library follows this file exactly.
+#include "frechet_oracle.h"
+
+#include <cfloat>
+#include <cmath>
+#include <cstdlib>
+#include <vector>
+#include <algorithm>
+
+namespace {
+
+template <typename T> struct Lim;
+template <> struct Lim<double> { static double big() { return (double)FLT_MAX; } };
+template <> struct Lim<float> { static float big() { return FLT_MAX; } };
+
+template <typename T>
+inline T fmaT(T a, T b, T c) { return std::fma(a, b, c); }
+
+template <typename T>
+inline T sqrtT(T a) { return std::sqrt(a); }
+
+// Squared Euclidean distance as the DFD kernel forms it (.cu:615-619).
+template <typename T>
+inline T dist2(const T *a, const T *b, int D) {
+    T acc = T(0);
+    for (int k = 0; k < D; ++k) {
+        T t = a[k] - b[k];
+        acc = fmaT(t, t, acc);
+    }
+    return acc;
+}
+
+// Squared distance from point x to segment [a, b] (.cu:375-394 right weight;
+// the top weight uses the same steps with the roles of the curves swapped,
+// README.md:79).
+template <typename T>
+inline T point_seg2(const T *x, const T *a, const T *b, int D) {
+    T ab2 = T(0), apab = T(0);
+    T ab[3];
+    for (int k = 0; k < D; ++k) {
+        ab[k] = b[k] - a[k];
+        T ap = x[k] - a[k];
+        ab2 = fmaT(ab[k], ab[k], ab2);
+        apab = fmaT(ap, ab[k], apab);
+    }
+    T t = apab / ab2;
+    t = std::fmin(T(1), std::fmax(T(0), t));
+    T w = T(0);
+    for (int k = 0; k < D; ++k) {
+        T q = fmaT(t, ab[k], a[k]);
+        T d = x[k] - q;
+        w = fmaT(d, d, w);
+    }
+    return w;
+}
+
+// Free interval of an edge: the part of segment [s0, s1] within sqrt(r2) of
+// vertex c (.cu:1173-1230).  A4 = 4*|s1-s0|^2 and inv2A = 0.5/|s1-s0|^2 are
+// per-segment terms.  mi = s0 - c, di = s1 - s0 (.cu:1178-1179, 1208-1209).
+template <typename T>
+inline void free_iv(const T *c, const T *s0, const T *s1, int D, T r2, T A4, T inv2A,
+                    T &lb, T &ub) {
+    T B = T(0), C = -r2;
+    for (int k = 0; k < D; ++k) {
+        T di = s1[k] - s0[k];
+        T mi = s0[k] - c[k];
+        B = fmaT(mi, di, B);
+        C = fmaT(mi, mi, C);
+    }
+    B = B * T(2);
+    T disc = fmaT(B, B, -(A4 * C));
+    lb = T(-1);
+    ub = T(-2);
+    if (disc >= T(0)) {
+        T s = sqrtT(disc);
+        T ta = (-B - s) * inv2A;
+        T tb = (-B + s) * inv2A;
+        bool invalid = tb < T(0) || ta > T(1);
+        lb = invalid ? T(-1) : std::fmax(ta, T(0));
+        ub = invalid ? T(-2) : std::fmin(tb, T(1));
+    }
+}
+
+// Squared length of a segment as the decision kernel forms it
+// (.cu:1001-1005 for P, .cu:1138-1141 for Q).
+template <typename T>
+inline T seg_len2(const T *a, const T *b, int D) {
+    T acc = T(0);
+    for (int k = 0; k < D; ++k) {
+        T d = b[k] - a[k];
+        acc = fmaT(d, d, acc);
+    }
+    return acc;
+}
+
+template <typename T>
+std::vector<T> to_t(const double *X, int64_t cnt) {
+    std::vector<T> v((size_t)cnt);
+    for (int64_t i = 0; i < cnt; ++i) v[(size_t)i] = (T)X[i];
+    return v;
+}
+
+// DFD<D> (.cu:458-641) row by row.  colv[j] carries right = min(top, bot) of
+// row i-1 into row i as "left" (.cu:624-626); (0,0) enters with left = 0,
+// row 0 with left = FLT_MAX, column 0 with bot = FLT_MAX (.cu:620-622).
+template <typename T>
+T dfd_impl(const T *P, int64_t m, const T *Q, int64_t n, int D) {
+    const T big = Lim<T>::big();
+    std::vector<T> colv((size_t)n, big);
+    T top = T(0);
+    for (int64_t i = 0; i < m; ++i) {
+        T bot = big;
+        const T *pi = P + i * D;
+        for (int64_t j = 0; j < n; ++j) {
+            T left = (i == 0) ? ((j == 0) ? T(0) : big) : colv[(size_t)j];
+            T d = dist2(pi, Q + j * D, D);
+            T in = std::fmin(left, bot);
+            top = std::fmax(d, in);
+            colv[(size_t)j] = std::fmin(top, bot);
+            bot = top;
+        }
+    }
+    return sqrtT(top);
+}
+
+// VEFD<D> (.cu:200-432).  Cell (i,j) pairs segment p_i p_{i+1} with q_j q_{j+1}.
+// T = max(w_top, min(bot, left)), R = max(w_right, ...) (.cu:373, 418-419);
+// row 0 starts with bot = init^2 (.cu:258-259), other rows with FLT_MAX;
+// result sqrt(min(T, R)) of cell (m-2, n-2) (.cu:426-429).
+template <typename T>
+T vefd_rows(const T *P, int64_t /*m*/, const T *Q, int64_t n, int D, T init,
+            int64_t r0, int64_t r1, std::vector<T> &colv, T *lastTR) {
+    const T big = Lim<T>::big();
+    const int64_t cols = n - 1;
+    T Tv = T(0), Rv = T(0);
+    for (int64_t i = r0; i < r1; ++i) {
+        T bot = (i == 0) ? init * init : big;
+        const T *pa = P + i * D, *pb = P + (i + 1) * D;
+        for (int64_t j = 0; j < cols; ++j) {
+            T left = (i == 0) ? big : colv[(size_t)j];
+            const T *qa = Q + j * D, *qb = Q + (j + 1) * D;
+            T in = std::fmin(left, bot);
+            T wr = point_seg2(pb, qa, qb, D);
+            T wt = point_seg2(qb, pa, pb, D);
+            Tv = std::fmax(wt, in);
+            Rv = std::fmax(wr, in);
+            bot = Tv;
+            colv[(size_t)j] = Rv;
+        }
+    }
+    if (lastTR) { lastTR[0] = Tv; lastTR[1] = Rv; }
+    return std::fmin(Tv, Rv);
+}
+
+template <typename T>
+T vefd_impl(const T *P, int64_t m, const T *Q, int64_t n, int D, T init) {
+    std::vector<T> colv((size_t)(n - 1), Lim<T>::big());
+    T v = vefd_rows(P, m, Q, n, D, init, 0, m - 1, colv, (T *)nullptr);
+    return sqrtT(v);
+}
+
+// reachable<D> (.cu:960-1256) row by row.  Only lower ends travel between
+// cells: rightLB to the next row (as leftLB), topLB along the row (as botLB)
+// (.cu:1231-1249).  Row 0 starts with botLB = 0, others with -1 (.cu:1026-1027);
+// the left boundary is -1 (.cu:656).  Returns rightUB of cell (m-2, n-2).
+template <typename T>
+T decide_rows(const T *P, int64_t m, const T *Q, int64_t n, int D, T r,
+              int64_t r0, int64_t r1, std::vector<T> &colv,
+              const std::vector<T> &A4b, const std::vector<T> &inv2b) {
+    (void)m;
+    const int64_t cols = n - 1;
+    const T r2 = r * r;
+    T rightUB = T(-2);
+    for (int64_t i = r0; i < r1; ++i) {
+        const T *pa = P + i * D, *pb = P + (i + 1) * D;
+        T Aa = seg_len2(pa, pb, D);
+        T A4a = T(4) * Aa, inv2a = T(0.5) / Aa;
+        T botLB = (i == 0) ? T(0) : T(-1);
+        for (int64_t j = 0; j < cols; ++j) {
+            T leftLB = (i == 0) ? T(-1) : colv[(size_t)j];
+            const T *qa = Q + j * D, *qb = Q + (j + 1) * D;
+            T rLB, rUB, tLB, tUB;
+            free_iv(pb, qa, qb, D, r2, A4b[(size_t)j], inv2b[(size_t)j], rLB, rUB);
+            free_iv(qb, pa, pb, D, r2, A4a, inv2a, tLB, tUB);
+            if (botLB < T(0)) {
+                bool inv = leftLB > rUB || leftLB < T(0);
+                T t2 = std::fmax(rLB, leftLB);
+                rLB = inv ? T(-1) : t2;
+                rUB = inv ? T(-2) : rUB;
+            }
+            if (leftLB < T(0)) {
+                bool inv = botLB > tUB || botLB < T(0);
+                T t2 = std::fmax(tLB, botLB);
+                tLB = inv ? T(-1) : t2;
+                tUB = inv ? T(-2) : tUB;
+            }
+            botLB = tLB;
+            colv[(size_t)j] = rLB;
+            rightUB = rUB;
+        }
+    }
+    return rightUB;
+}
+
+template <typename T>
+void q_seg_terms(const T *Q, int64_t n, int D, std::vector<T> &A4b, std::vector<T> &inv2b) {
+    A4b.resize((size_t)(n - 1));
+    inv2b.resize((size_t)(n - 1));
+    for (int64_t j = 0; j + 1 < n; ++j) {
+        // .cu:1138-1141 forms q_j - q_{j+1}; its square equals that of
+        // q_{j+1} - q_j bit for bit.
+        T Ab = seg_len2(Q + j * D, Q + (j + 1) * D, D);
+        A4b[(size_t)j] = T(4) * Ab;
+        inv2b[(size_t)j] = T(0.5) / Ab;
+    }
+}
+
+template <typename T>
+T decide_impl(const T *P, int64_t m, const T *Q, int64_t n, int D, T r) {
+    std::vector<T> A4b, inv2b;
+    q_seg_terms(Q, n, D, A4b, inv2b);
+    std::vector<T> colv((size_t)(n - 1), T(-1));
+    return decide_rows(P, m, Q, n, D, r, 0, m - 1, colv, A4b, inv2b);
+}
+
+inline bool continue_bisect(double lb, double ub, int stop_mode, double eps) {
+    return stop_mode == 0 ? (ub - lb > eps) : (ub - lb > eps * ub);
+}
+
+}  // namespace
+
+extern "C" {
+
+double or_endpoint_lb(const double *P, int64_t m, const double *Q, int64_t n, int dims) {
+    // Point::dist_sqr (geometry_basics.hpp:72-81): host code, no FMA.
+    auto d2 = [&](const double *a, const double *b) {
+        double result = 0.0;
+        for (int k = 0; k < dims; ++k) {
+            double ds = a[k] - b[k];
+            double sq = ds * ds;
+            result = result + sq;
+        }
+        return result;
+    };
+    double a = d2(P, Q);
+    double b = d2(P + (m - 1) * dims, Q + (n - 1) * dims);
+    return std::sqrt(std::max(a, b));
+}
+
+double or_dfd(const double *P, int64_t m, const double *Q, int64_t n, int dims) {
+    return dfd_impl<double>(P, m, Q, n, dims);
+}
+
+double or_vefd(const double *P, int64_t m, const double *Q, int64_t n, int dims,
+               double init_lb) {
+    return vefd_impl<double>(P, m, Q, n, dims, init_lb);
+}
+
+double or_vefd_f32(const double *P, int64_t m, const double *Q, int64_t n, int dims,
+                   double init_lb) {
+    auto Pf = to_t<float>(P, m * dims);
+    auto Qf = to_t<float>(Q, n * dims);
+    return (double)vefd_impl<float>(Pf.data(), m, Qf.data(), n, dims, (float)init_lb);
+}
+
+double or_decide_ub(const double *P, int64_t m, const double *Q, int64_t n, int dims,
+                    double r) {
+    return decide_impl<double>(P, m, Q, n, dims, r);
+}
+
+int or_decide(const double *P, int64_t m, const double *Q, int64_t n, int dims, double r) {
+    return or_decide_ub(P, m, Q, n, dims, r) == 1.0 ? 1 : 0;
+}
+
+double or_decide_ub_f32(const double *P, int64_t m, const double *Q, int64_t n, int dims,
+                        double r) {
+    auto Pf = to_t<float>(P, m * dims);
+    auto Qf = to_t<float>(Q, n * dims);
+    return (double)decide_impl<float>(Pf.data(), m, Qf.data(), n, dims, (float)r);
+}
+
+void or_free_interval(const double *P, const double *Q, int dims, int64_t i, int64_t j,
+                      double r, int which, double *out) {
+    double lb, ub;
+    if (which == 0) {
+        const double *qa = Q + j * dims, *qb = Q + (j + 1) * dims;
+        double Ab = seg_len2(qa, qb, dims);
+        free_iv(P + (i + 1) * dims, qa, qb, dims, r * r, 4.0 * Ab, 0.5 / Ab, lb, ub);
+    } else {
+        const double *pa = P + i * dims, *pb = P + (i + 1) * dims;
+        double Aa = seg_len2(pa, pb, dims);
+        free_iv(Q + (j + 1) * dims, pa, pb, dims, r * r, 4.0 * Aa, 0.5 / Aa, lb, ub);
+    }
+    out[0] = lb;
+    out[1] = ub;
+}
+
+void or_ve_weights(const double *P, const double *Q, int dims, int64_t i, int64_t j,
+                   double *out) {
+    const double *pa = P + i * dims, *pb = P + (i + 1) * dims;
+    const double *qa = Q + j * dims, *qb = Q + (j + 1) * dims;
+    out[0] = point_seg2(qb, pa, pb, dims);
+    out[1] = point_seg2(pb, qa, qb, dims);
+}
+
+int64_t or_distance(const double *P, int64_t m, const double *Q, int64_t n, int dims,
+                    int stop_mode, double eps, int vefd_gate, double *out,
+                    double *splits, int32_t *decisions, int64_t max_trace) {
+    double lb0 = or_endpoint_lb(P, m, Q, n, dims);
+    double ub = or_dfd(P, m, Q, n, dims);
+    double lb = lb0;
+    out[1] = lb0;
+    out[2] = ub;
+    if (m < 2 || n < 2) {
+        // A single point against a curve: the distance is the DFD exactly.
+        out[0] = ub;
+        out[3] = lb0;
+        out[4] = ub;
+        out[5] = ub;
+        return 0;
+    }
+    if (!vefd_gate || m * n > 10000) lb = or_vefd(P, m, Q, n, dims, lb0);
+    out[3] = lb;
+    int64_t iter = 0;
+    std::vector<double> A4b, inv2b;
+    q_seg_terms(Q, n, dims, A4b, inv2b);
+    std::vector<double> colv((size_t)(n - 1));
+    while (continue_bisect(lb, ub, stop_mode, eps)) {
+        double split = (ub + lb) / 2.0;
+        std::fill(colv.begin(), colv.end(), -1.0);
+        double rub = decide_rows<double>(P, m, Q, n, dims, split, 0, m - 1, colv, A4b, inv2b);
+        int dec = rub == 1.0;
+        if (splits && iter < max_trace) splits[iter] = split;
+        if (decisions && iter < max_trace) decisions[iter] = dec;
+        ++iter;
+        if (dec) ub = split; else lb = split;
+    }
+    out[0] = (ub + lb) / 2.0;
+    out[4] = lb;
+    out[5] = ub;
+    return iter;
+}
+
+double or_sweep_rows(const double *P, int64_t m, const double *Q, int64_t n, int dims,
+                     int kind, double r, int64_t r0, int64_t r1) {
+    if (kind == 0) {
+        std::vector<double> colv((size_t)(n - 1), (double)FLT_MAX);
+        double tr[2];
+        vefd_rows<double>(P, m, Q, n, dims, r, r0, r1, colv, tr);
+        double s = 0;
+        for (double v : colv) s += v;
+        return s;
+    }
+    if (kind == 1) {
+        std::vector<double> A4b, inv2b;
+        q_seg_terms(Q, n, dims, A4b, inv2b);
+        std::vector<double> colv((size_t)(n - 1), 0.0);
+        decide_rows<double>(P, m, Q, n, dims, r, r0, r1, colv, A4b, inv2b);
+        double s = 0;
+        for (double v : colv) s += v;
+        return s;
+    }
+    return dfd_impl<double>(P + r0 * dims, r1 - r0, Q, n, dims);
+}
+
+}  // extern "C"

@@ -1,0 +1,475 @@
+// C ABI of the gfx950 continuous Frechet distance library (include/rfd.h).
+//
+// Host side of the path: sessions hold both curves resident in HBM, the
+// bound sweeps (DFD, VE-FD) and the decision sweep run as band-wavefront
+// kernels (sweep_kernels.hpp), and the bisection of frechet.cpp:18-75 runs on
+// the host between decision sweeps.
+#include "rfd.h"
+#include "sweep_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+struct rfd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int cu_count = 0;
+    int max_waves = 0;
+    rfd_stats stats{};
+    char err[512] = {0};
+};
+
+struct rfd_session {
+    rfd_ctx *ctx = nullptr;
+    int64_t m = 0, n = 0;
+    int dims = 0;
+    double *dP = nullptr, *dQ = nullptr;
+    float *fP = nullptr, *fQ = nullptr;
+    double *qseg64 = nullptr;
+    float *qseg32 = nullptr;
+    void *slots = nullptr;
+    int *progress = nullptr;
+    int *queue = nullptr;
+    double *result = nullptr;
+    double endpoint_lb = 0.0;
+};
+
+namespace {
+
+void set_err(rfd_ctx *ctx, const char *fmt, ...) {
+    if (!ctx) return;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(ctx->err, sizeof(ctx->err), fmt, ap);
+    va_end(ap);
+}
+
+#define RFD_HIP(ctx, call)                                                          \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            set_err((ctx), "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
+                    __FILE__, __LINE__);                                            \
+            return RFD_ERR_HIP;                                                     \
+        }                                                                           \
+    } while (0)
+
+// frechet.cpp:30 with Point::dist_sqr (geometry_basics.hpp:72-81): host
+// arithmetic, each product rounded before the sum.
+double endpoint_bound(const double *P, int64_t m, const double *Q, int64_t n, int dims) {
+    auto d2 = [&](const double *a, const double *b) {
+        double result = 0.0;
+        for (int k = 0; k < dims; ++k) {
+            volatile double ds = a[k] - b[k];
+            volatile double sq = ds * ds;
+            result = result + sq;
+        }
+        return result;
+    };
+    double a = d2(P, Q);
+    double b = d2(P + (m - 1) * dims, Q + (n - 1) * dims);
+    return std::sqrt(std::max(a, b));
+}
+
+int64_t nbands_for(int64_t rows) { return (rows + 63) / 64; }
+
+int grid_for(rfd_ctx *ctx, int64_t nb) {
+    int64_t g = std::min<int64_t>(nb, ctx->max_waves);
+    return (int)std::max<int64_t>(g, 1);
+}
+
+template <int OP, typename T>
+hipError_t launch_sweep(int dims, int grid, hipStream_t st, const rfd::SweepArgs &a) {
+    if (dims == 2)
+        hipLaunchKernelGGL((rfd::band_sweep<OP, 2, T>), dim3(grid), dim3(64), 0, st, a);
+    else
+        hipLaunchKernelGGL((rfd::band_sweep<OP, 3, T>), dim3(grid), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+rfd_status ensure_f32(rfd_session *s) {
+    rfd_ctx *ctx = s->ctx;
+    if (s->fP) return RFD_OK;
+    RFD_HIP(ctx, hipMalloc(&s->fP, sizeof(float) * s->m * s->dims));
+    RFD_HIP(ctx, hipMalloc(&s->fQ, sizeof(float) * s->n * s->dims));
+    long long cp = s->m * s->dims, cq = s->n * s->dims;
+    hipLaunchKernelGGL(rfd::to_f32, dim3((unsigned)((cp + 255) / 256)), dim3(256), 0, ctx->stream,
+                       s->dP, s->fP, cp);
+    hipLaunchKernelGGL(rfd::to_f32, dim3((unsigned)((cq + 255) / 256)), dim3(256), 0, ctx->stream,
+                       s->dQ, s->fQ, cq);
+    RFD_HIP(ctx, hipGetLastError());
+    return RFD_OK;
+}
+
+template <typename T>
+rfd_status ensure_qseg(rfd_session *s, T **out, const T *Qd) {
+    rfd_ctx *ctx = s->ctx;
+    if (*out) return RFD_OK;
+    long long cols = s->n - 1;
+    RFD_HIP(ctx, hipMalloc(out, sizeof(T) * 2 * cols));
+    unsigned g = (unsigned)((cols + 255) / 256);
+    if (s->dims == 2)
+        hipLaunchKernelGGL((rfd::qseg_terms<2, T>), dim3(g), dim3(256), 0, ctx->stream, Qd, cols, *out);
+    else
+        hipLaunchKernelGGL((rfd::qseg_terms<3, T>), dim3(g), dim3(256), 0, ctx->stream, Qd, cols, *out);
+    RFD_HIP(ctx, hipGetLastError());
+    return RFD_OK;
+}
+
+// One sweep: init kernel + band kernel, timed with HIP events on the
+// context's stream, result copied back.
+template <int OP, typename T>
+rfd_status run_sweep(rfd_session *s, const T *P, const T *Q, const T *qseg, double param,
+                     rfd_kernel_kind kind, double *result) {
+    rfd_ctx *ctx = s->ctx;
+    const int64_t rows = (OP == rfd::OP_DFD) ? s->m : s->m - 1;
+    const int64_t cols = (OP == rfd::OP_DFD) ? s->n : s->n - 1;
+    const int64_t nb = nbands_for(rows);
+    rfd::SweepArgs a{};
+    a.P = P;
+    a.Q = Q;
+    a.qseg = qseg;
+    a.slots = s->slots;
+    a.progress = s->progress;
+    a.queue = s->queue;
+    a.result = s->result;
+    a.rows = rows;
+    a.cols = cols;
+    a.nbands = (int)nb;
+    a.param = param;
+    const double sentinel = -3.0;  // .cu:193 (result = -3 until the terminal cell writes it)
+    RFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    hipLaunchKernelGGL(rfd::sweep_init, dim3((unsigned)std::min<int64_t>((nb + 255) / 256, 1024)),
+                       dim3(256), 0, ctx->stream, s->progress, (int)nb, s->queue, s->result,
+                       sentinel);
+    RFD_HIP(ctx, hipGetLastError());
+    RFD_HIP(ctx, (launch_sweep<OP, T>(s->dims, grid_for(ctx, nb), ctx->stream, a)));
+    RFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    double h = 0;
+    RFD_HIP(ctx, hipMemcpyAsync(&h, s->result, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    RFD_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.launches[kind] += 1;
+    ctx->stats.ms[kind] += ms;
+    ctx->stats.cells[kind] += (double)rows * (double)cols;
+    if (h == sentinel) {
+        set_err(ctx, "sweep kernel %d wrote no terminal value", (int)kind);
+        return RFD_ERR_STATE;
+    }
+    *result = h;
+    return RFD_OK;
+}
+
+bool keep_bisecting(double lb, double ub, const rfd_opts &o, double eps) {
+    return o.stop_mode == RFD_STOP_ABS ? (ub - lb > eps) : (ub - lb > eps * ub);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *rfd_status_string(rfd_status s) {
+    switch (s) {
+        case RFD_OK: return "ok";
+        case RFD_ERR_ARG: return "invalid argument";
+        case RFD_ERR_HIP: return "HIP runtime error";
+        case RFD_ERR_NO_DEVICE: return "no GPU device";
+        case RFD_ERR_STATE: return "inconsistent sweep result";
+        case RFD_ERR_NOMEM: return "device allocation failed";
+    }
+    return "unknown status";
+}
+
+const char *rfd_last_error(rfd_ctx *ctx) { return ctx ? ctx->err : "null context"; }
+
+void rfd_default_opts(rfd_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->stop_mode = RFD_STOP_ABS;
+    o->precision = RFD_PREC_FP64;
+    o->eps = (double)FLT_EPSILON;  // compute_distance_wrapper.cpp:6
+    o->vefd_gate = 1;
+    o->verbose = 0;
+    o->mixed_switch = 1e-4;
+}
+
+rfd_status rfd_create(rfd_ctx **out, int device) {
+    if (!out) return RFD_ERR_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return RFD_ERR_NO_DEVICE;
+    if (device < 0 || device >= count) return RFD_ERR_ARG;
+    rfd_ctx *ctx = new rfd_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete ctx;
+        return RFD_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        delete ctx;
+        return RFD_ERR_HIP;
+    }
+    ctx->cu_count = prop.multiProcessorCount;
+    int waves_per_cu = 8;
+    if (const char *e = std::getenv("RFD_WAVES_PER_CU")) waves_per_cu = std::max(1, std::atoi(e));
+    ctx->max_waves = ctx->cu_count * waves_per_cu;
+    // Absolute cap, used by tests to make waves take several bands each.
+    if (const char *e = std::getenv("RFD_MAX_WAVES")) ctx->max_waves = std::max(1, std::atoi(e));
+    *out = ctx;
+    return RFD_OK;
+}
+
+rfd_status rfd_destroy(rfd_ctx *ctx) {
+    if (!ctx) return RFD_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RFD_OK;
+}
+
+rfd_status rfd_get_stats(rfd_ctx *ctx, rfd_stats *out) {
+    if (!ctx || !out) return RFD_ERR_ARG;
+    *out = ctx->stats;
+    return RFD_OK;
+}
+
+rfd_status rfd_reset_stats(rfd_ctx *ctx) {
+    if (!ctx) return RFD_ERR_ARG;
+    std::memset(&ctx->stats, 0, sizeof(ctx->stats));
+    return RFD_OK;
+}
+
+rfd_status rfd_session_destroy(rfd_session *s) {
+    if (!s) return RFD_ERR_ARG;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    (void)hipFree(s->dP);
+    (void)hipFree(s->dQ);
+    (void)hipFree(s->fP);
+    (void)hipFree(s->fQ);
+    (void)hipFree(s->qseg64);
+    (void)hipFree(s->qseg32);
+    (void)hipFree(s->slots);
+    (void)hipFree(s->progress);
+    (void)hipFree(s->queue);
+    (void)hipFree(s->result);
+    delete s;
+    return RFD_OK;
+}
+
+rfd_status rfd_session_create(rfd_ctx *ctx, const double *P, int64_t m, const double *Q,
+                              int64_t n, int dims, rfd_session **out) {
+    if (!ctx || !P || !Q || !out) return RFD_ERR_ARG;
+    *out = nullptr;
+    if (m < 1 || n < 1 || (dims != 2 && dims != 3)) {
+        set_err(ctx, "need non-empty curves and dims in {2,3} (m=%lld n=%lld dims=%d)",
+                (long long)m, (long long)n, dims);
+        return RFD_ERR_ARG;
+    }
+    if (m > (int64_t)1 << 30 || n > (int64_t)1 << 30) {
+        set_err(ctx, "curves longer than 2^30 points are not supported");
+        return RFD_ERR_ARG;
+    }
+    RFD_HIP(ctx, hipSetDevice(ctx->device));
+    rfd_session *s = new rfd_session();
+    s->ctx = ctx;
+    s->m = m;
+    s->n = n;
+    s->dims = dims;
+    s->endpoint_lb = endpoint_bound(P, m, Q, n, dims);
+    auto fail = [&](hipError_t e, const char *what) {
+        set_err(ctx, "%s failed: %s", what, hipGetErrorString(e));
+        rfd_session_destroy(s);
+        return e == hipErrorOutOfMemory ? RFD_ERR_NOMEM : RFD_ERR_HIP;
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&s->dP, sizeof(double) * m * dims)) != hipSuccess) return fail(e, "hipMalloc(P)");
+    if ((e = hipMalloc(&s->dQ, sizeof(double) * n * dims)) != hipSuccess) return fail(e, "hipMalloc(Q)");
+    // Band-boundary buffers: two columns of n doubles; progress flags: one per band.
+    if ((e = hipMalloc(&s->slots, sizeof(double) * 2 * n)) != hipSuccess) return fail(e, "hipMalloc(slots)");
+    if ((e = hipMalloc(&s->progress, sizeof(int) * (nbands_for(m) + 1))) != hipSuccess)
+        return fail(e, "hipMalloc(progress)");
+    if ((e = hipMalloc(&s->queue, sizeof(int))) != hipSuccess) return fail(e, "hipMalloc(queue)");
+    if ((e = hipMalloc(&s->result, sizeof(double))) != hipSuccess) return fail(e, "hipMalloc(result)");
+    if ((e = hipMemcpyAsync(s->dP, P, sizeof(double) * m * dims, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
+        return fail(e, "hipMemcpy(P)");
+    if ((e = hipMemcpyAsync(s->dQ, Q, sizeof(double) * n * dims, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
+        return fail(e, "hipMemcpy(Q)");
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return fail(e, "hipStreamSynchronize");
+    *out = s;
+    return RFD_OK;
+}
+
+rfd_status rfd_session_bounds(rfd_session *s, double *dfd, double *vefd, int run_vefd, int fp32) {
+    if (!s || !dfd || !vefd) return RFD_ERR_ARG;
+    rfd_ctx *ctx = s->ctx;
+    RFD_HIP(ctx, hipSetDevice(ctx->device));
+    rfd_status st = run_sweep<rfd::OP_DFD, double>(s, s->dP, s->dQ, nullptr, 0.0, RFD_K_DFD, dfd);
+    if (st != RFD_OK) return st;
+    if (run_vefd && s->m >= 2 && s->n >= 2) {
+        double init = *vefd;
+        if (fp32) {
+            if ((st = ensure_f32(s)) != RFD_OK) return st;
+            st = run_sweep<rfd::OP_VEFD, float>(s, s->fP, s->fQ, nullptr, init, RFD_K_VEFD32, vefd);
+        } else {
+            st = run_sweep<rfd::OP_VEFD, double>(s, s->dP, s->dQ, nullptr, init, RFD_K_VEFD64, vefd);
+        }
+        if (st != RFD_OK) return st;
+    }
+    return RFD_OK;
+}
+
+rfd_status rfd_session_decide(rfd_session *s, double radius, int fp32, int *reachable,
+                              double *terminal_ub) {
+    if (!s || !reachable) return RFD_ERR_ARG;
+    rfd_ctx *ctx = s->ctx;
+    if (s->m < 2 || s->n < 2) {
+        set_err(ctx, "the decision sweep needs at least one segment per curve");
+        return RFD_ERR_ARG;
+    }
+    RFD_HIP(ctx, hipSetDevice(ctx->device));
+    double ub = 0;
+    rfd_status st;
+    if (fp32) {
+        if ((st = ensure_f32(s)) != RFD_OK) return st;
+        if ((st = ensure_qseg<float>(s, &s->qseg32, s->fQ)) != RFD_OK) return st;
+        st = run_sweep<rfd::OP_DEC, float>(s, s->fP, s->fQ, s->qseg32, radius, RFD_K_DEC32, &ub);
+    } else {
+        if ((st = ensure_qseg<double>(s, &s->qseg64, s->dQ)) != RFD_OK) return st;
+        st = run_sweep<rfd::OP_DEC, double>(s, s->dP, s->dQ, s->qseg64, radius, RFD_K_DEC64, &ub);
+    }
+    if (st != RFD_OK) return st;
+    *reachable = (ub == 1.0) ? 1 : 0;  // .cu:1592
+    if (terminal_ub) *terminal_ub = ub;
+    return RFD_OK;
+}
+
+rfd_status rfd_session_distance(rfd_session *s, const rfd_opts *opts, rfd_result *res) {
+    if (!s || !res) return RFD_ERR_ARG;
+    rfd_opts o;
+    if (opts) o = *opts; else rfd_default_opts(&o);
+    rfd_ctx *ctx = s->ctx;
+    std::memset(res, 0, sizeof(*res));
+    RFD_HIP(ctx, hipSetDevice(ctx->device));
+    hipEvent_t t0, t1;
+    RFD_HIP(ctx, hipEventCreate(&t0));
+    RFD_HIP(ctx, hipEventCreate(&t1));
+    RFD_HIP(ctx, hipEventRecord(t0, ctx->stream));
+    if (o.verbose) std::printf("%lld, %lld\n", (long long)s->m, (long long)s->n);  // frechet.cpp:21
+
+    double lb = s->endpoint_lb;
+    double ub = 0;
+    res->endpoint_lb = lb;
+    if (o.verbose) std::printf("Naive LB: %.14f\n", lb);  // frechet.cpp:31
+    const bool degenerate = s->m < 2 || s->n < 2;
+    const bool run_vefd = !degenerate && (!o.vefd_gate || s->m * s->n > 10000);
+    const bool mixed = o.precision == RFD_PREC_MIXED;
+    double vefd = lb;
+    rfd_status st = rfd_session_bounds(s, &ub, &vefd, run_vefd ? 1 : 0, mixed ? 1 : 0);
+    if (st != RFD_OK) return st;
+    res->dfd = ub;
+    if (degenerate) {
+        // One point against a curve: the DFD is the distance.
+        lb = ub;
+    } else {
+        lb = vefd;
+    }
+    res->vefd = lb;
+    if (o.verbose) std::printf("LB: %.14f\nUB: %.14f\n", lb, ub);  // frechet.cpp:34
+    const double eps = o.eps;
+    int64_t iter = 0, iter32 = 0, cert = 0;
+    if (mixed && !degenerate) {
+        // FP32 VE-FD is an estimate, not a certified bound: clamp to [endpoint, DFD].
+        lb = std::min(std::max(lb, s->endpoint_lb), ub);
+        // FP32 bisection while the bracket is wide.
+        while (keep_bisecting(lb, ub, o, eps) && (ub - lb) > o.mixed_switch * ub) {
+            double split = (ub + lb) / 2.0;
+            int r = 0;
+            if ((st = rfd_session_decide(s, split, 1, &r, nullptr)) != RFD_OK) return st;
+            ++iter;
+            ++iter32;
+            if (r) ub = split; else lb = split;
+        }
+        // Certify the FP32 bracket with FP64 decisions at its ends: lb must be
+        // unreachable and ub reachable.  Widen on failure.
+        if (keep_bisecting(lb, ub, o, eps)) {
+            double lo = lb, hi = ub;
+            double w = std::max(hi - lo, 1e-12 * std::max(1.0, hi));
+            for (int guard = 0; guard < 64; ++guard) {
+                int r = 0;
+                if (lo > s->endpoint_lb) {
+                    if ((st = rfd_session_decide(s, lo, 0, &r, nullptr)) != RFD_OK) return st;
+                    ++cert;
+                    if (r) { hi = lo; lo = std::max(s->endpoint_lb, lo - w); w *= 2; continue; }
+                }
+                break;
+            }
+            for (int guard = 0; guard < 64; ++guard) {
+                int r = 0;
+                if (hi < res->dfd) {
+                    if ((st = rfd_session_decide(s, hi, 0, &r, nullptr)) != RFD_OK) return st;
+                    ++cert;
+                    if (!r) { lo = hi; hi = std::min(res->dfd, hi + w); w *= 2; continue; }
+                }
+                break;
+            }
+            lb = lo;
+            ub = hi;
+        }
+    }
+    while (keep_bisecting(lb, ub, o, eps)) {  // frechet.cpp:41-60
+        double split = (ub + lb) / 2.0;
+        int r = 0;
+        if ((st = rfd_session_decide(s, split, 0, &r, nullptr)) != RFD_OK) return st;
+        ++iter;
+        if (r) ub = split; else lb = split;
+    }
+    double value = (ub + lb) / 2.0;  // frechet.cpp:63
+    RFD_HIP(ctx, hipEventRecord(t1, ctx->stream));
+    RFD_HIP(ctx, hipEventSynchronize(t1));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    res->distance = value;
+    res->lb = lb;
+    res->ub = ub;
+    res->err_bound = value > 0 ? (ub - lb) / 2.0 / value : 0.0;
+    res->iterations = iter;
+    res->iterations_fp32 = iter32;
+    res->certify_steps = cert;
+    res->ms_total = ms;
+    if (o.verbose) {
+        std::printf("Iteration: %lld\n", (long long)iter);  // frechet.cpp:65-66
+        std::printf("Result: %.7f\n", value);
+        std::printf("Total Time: %f ms\n", (double)ms);
+    }
+    return RFD_OK;
+}
+
+rfd_status rfd_distance(rfd_ctx *ctx, const double *P, int64_t m, const double *Q, int64_t n,
+                        int dims, const rfd_opts *opts, rfd_result *res) {
+    rfd_session *s = nullptr;
+    rfd_status st = rfd_session_create(ctx, P, m, Q, n, dims, &s);
+    if (st != RFD_OK) return st;
+    st = rfd_session_distance(s, opts, res);
+    rfd_session_destroy(s);
+    return st;
+}
+
+}  // extern "C"
