@@ -65,6 +65,14 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # The reference CLI (fd <c1> <c2> <dim> [1]) linked against the library.
+    main = os.path.join(CSRC, "fd_main.cpp")
+    if os.path.exists(main):
+        cmd = [hipcc(), "-O2", "-std=c++17", f"-I{INCLUDE}", main, "-o", os.path.join(PKG, "fd"),
+               f"-L{PKG}", "-lrfd", f"-Wl,-rpath,$ORIGIN"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
     return LIB
 
 

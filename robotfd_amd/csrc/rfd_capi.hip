@@ -223,7 +223,7 @@ rfd_status rfd_create(rfd_ctx **out, int device) {
         return RFD_ERR_HIP;
     }
     ctx->cu_count = prop.multiProcessorCount;
-    int waves_per_cu = 8;
+    int waves_per_cu = 16;
     if (const char *e = std::getenv("RFD_WAVES_PER_CU")) waves_per_cu = std::max(1, std::atoi(e));
     ctx->max_waves = ctx->cu_count * waves_per_cu;
     // Absolute cap, used by tests to make waves take several bands each.

@@ -154,15 +154,26 @@ struct SweepArgs {
     double param;        // VE-FD: endpoint bound; decision: radius
 };
 
+// Q points (and, for the decision, the per-column terms) reach the lanes
+// through a ring of four 64-point chunks in LDS, structure-of-arrays so that
+// consecutive lanes read consecutive addresses.  Chunk k+2 is loaded into
+// registers at step 64k and written to LDS at step 64(k+1), so global-memory
+// latency stays off the per-step path.  At step s lane l reads point s-l+1
+// (s-l for DFD): chunks k-1, k, k+1 are live, the fourth slot is free.
 template <int OP, int D, typename T>
 __global__ __launch_bounds__(64) void band_sweep(SweepArgs a) {
     using N = Num<T>;
+    constexpr int NS = (OP == OP_DEC) ? 2 : 0;  // per-column terms staged
+    __shared__ T qring[4][D][64];
+    __shared__ T sring[4][NS > 0 ? NS : 1][64];
     const int lane = (int)threadIdx.x;
     const T *P = (const T *)a.P;
     const T *Q = (const T *)a.Q;
     const T *qseg = (const T *)a.qseg;
     T *slots = (T *)a.slots;
     const long long rows = a.rows, cols = a.cols;
+    // Points the sweep reads: DFD reads q_j (j < cols = n), the others q_j, q_{j+1}.
+    const long long npts = (OP == OP_DFD) ? cols : cols + 1;
     const T big = N::big();
     const T par = (T)a.param;
     const T r2 = par * par;
@@ -194,6 +205,29 @@ __global__ __launch_bounds__(64) void band_sweep(SweepArgs a) {
         else bot = (i == 0) ? T(0) : T(-1);
         T out = (OP == OP_DEC) ? T(-1) : big;
 
+        // Q staging: chunk 0 straight into LDS, chunk 1 into registers.
+        T pre[D], pres[NS > 0 ? NS : 1];
+        {
+            const long long p0 = lane;
+#pragma unroll
+            for (int k = 0; k < D; ++k) qring[0][k][lane] = (p0 < npts) ? Q[p0 * D + k] : T(0);
+            if (NS > 0) {
+                sring[0][0][lane] = (p0 < cols) ? qseg[2 * p0] : T(0);
+                sring[0][NS - 1][lane] = (p0 < cols) ? qseg[2 * p0 + 1] : T(0);
+            }
+            const long long p1 = 64 + lane;
+#pragma unroll
+            for (int k = 0; k < D; ++k) pre[k] = (p1 < npts) ? Q[p1 * D + k] : T(0);
+            if (NS > 0) {
+                pres[0] = (p1 < cols) ? qseg[2 * p1] : T(0);
+                pres[NS - 1] = (p1 < cols) ? qseg[2 * p1 + 1] : T(0);
+            }
+        }
+        // q_j of the lane's current column (VE-FD / decision); every lane starts at j = 0.
+        T qa[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) qa[k] = Q[k];
+
         const bool has_pred = band > 0;
         const bool has_succ = band + 1 < a.nbands;
         const T *sin = slots + (long long)((band + 1) & 1) * cols;
@@ -203,13 +237,31 @@ __global__ __launch_bounds__(64) void band_sweep(SweepArgs a) {
 
         for (long long s = 0; s < nsteps; ++s) {
             const long long j = s - lane;
-            if (has_pred && (s & 63) == 0 && s < cols) {
-                const long long need = (s + 64 < cols) ? s + 64 : cols;
-                while (__hip_atomic_load(&a.progress[band - 1], __ATOMIC_ACQUIRE,
-                                         __HIP_MEMORY_SCOPE_AGENT) < (int)need)
-                    __builtin_amdgcn_s_sleep(1);
-                const long long c = s + lane;
-                bin = (c < cols) ? sin[c] : T(0);
+            if ((s & 63) == 0) {
+                // Stage chunk k+1 (in registers) and fetch chunk k+2.
+                const long long kc = s >> 6;
+                const int slot = (int)((kc + 1) & 3);
+#pragma unroll
+                for (int k = 0; k < D; ++k) qring[slot][k][lane] = pre[k];
+                if (NS > 0) {
+                    sring[slot][0][lane] = pres[0];
+                    sring[slot][NS - 1][lane] = pres[NS - 1];
+                }
+                const long long pn = (kc + 2) * 64 + lane;
+#pragma unroll
+                for (int k = 0; k < D; ++k) pre[k] = (pn < npts) ? Q[pn * D + k] : T(0);
+                if (NS > 0) {
+                    pres[0] = (pn < cols) ? qseg[2 * pn] : T(0);
+                    pres[NS - 1] = (pn < cols) ? qseg[2 * pn + 1] : T(0);
+                }
+                if (has_pred && s < cols) {
+                    const long long need = (s + 64 < cols) ? s + 64 : cols;
+                    while (__hip_atomic_load(&a.progress[band - 1], __ATOMIC_ACQUIRE,
+                                             __HIP_MEMORY_SCOPE_AGENT) < (int)need)
+                        __builtin_amdgcn_s_sleep(1);
+                    const long long c = s + lane;
+                    bin = (c < cols) ? sin[c] : T(0);
+                }
             }
             T bval;
             if (has_pred) bval = lane_read(bin, (int)(s & 63));
@@ -221,23 +273,20 @@ __global__ __launch_bounds__(64) void band_sweep(SweepArgs a) {
 
             if (row_ok && j >= 0 && j < cols) {
                 const bool last = (i == rows - 1) && (j == cols - 1);
-                if (OP == OP_DFD) {
-                    T q[D];
+                // LDS index of the point this lane reads this step.
+                const long long pt = (OP == OP_DFD) ? j : j + 1;
+                const int rs = (int)((pt >> 6) & 3), ro = (int)(pt & 63);
+                T qb[D];
 #pragma unroll
-                    for (int k = 0; k < D; ++k) q[k] = Q[j * D + k];
-                    T d = dist2<T, D>(pa, q);
+                for (int k = 0; k < D; ++k) qb[k] = qring[rs][k][ro];
+                if (OP == OP_DFD) {
+                    T d = dist2<T, D>(pa, qb);
                     T in = N::min(left, bot);
                     T top = N::max(d, in);
                     out = N::min(top, bot);
                     bot = top;
                     if (last) *a.result = (double)N::sqrt(top);
                 } else if (OP == OP_VEFD) {
-                    T qa[D], qb[D];
-#pragma unroll
-                    for (int k = 0; k < D; ++k) {
-                        qa[k] = Q[j * D + k];
-                        qb[k] = Q[(j + 1) * D + k];
-                    }
                     T in = N::min(left, bot);
                     T wr = point_seg2<T, D>(pb, qa, qb);
                     T wt = point_seg2<T, D>(qb, pa, pb);
@@ -250,13 +299,8 @@ __global__ __launch_bounds__(64) void band_sweep(SweepArgs a) {
                     const T leftLB = left, botLB = bot;
                     T rLB = T(-1), rUB = T(-2), tLB = T(-1);
                     if (leftLB >= T(0) || botLB >= T(0)) {
-                        T qa[D], qb[D];
-#pragma unroll
-                        for (int k = 0; k < D; ++k) {
-                            qa[k] = Q[j * D + k];
-                            qb[k] = Q[(j + 1) * D + k];
-                        }
-                        const T A4b = qseg[2 * j], inv2b = qseg[2 * j + 1];
+                        const int ss = (int)((j >> 6) & 3), so = (int)(j & 63);
+                        const T A4b = sring[ss][0][so], inv2b = sring[ss][NS - 1][so];
                         T tUB;
                         free_iv<T, D>(pb, qa, qb, r2, A4b, inv2b, rLB, rUB);
                         free_iv<T, D>(qb, pa, pb, r2, A4a, inv2a, tLB, tUB);
@@ -275,6 +319,10 @@ __global__ __launch_bounds__(64) void band_sweep(SweepArgs a) {
                     bot = tLB;
                     out = rLB;
                     if (last) *a.result = (double)rUB;
+                }
+                if (OP != OP_DFD) {
+#pragma unroll
+                    for (int k = 0; k < D; ++k) qa[k] = qb[k];
                 }
             }
 
