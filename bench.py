@@ -105,12 +105,16 @@ def main() -> int:
     dom = max((k for k in stats if k in FLOPS_PER_CELL), key=lambda k: stats[k]["ms"])
     st = stats[dom]
     avg_ms = st["ms"] / st["launches"]
-    flops = FLOPS_PER_CELL[dom](dims) * st["cells"] / st["launches"]
+    # Flops of the cells the kernel actually evaluated (pruned / unreachable
+    # cells are skipped and not counted).
+    flops = FLOPS_PER_CELL[dom](dims) * st["evaluated"] / st["launches"]
     achieved = flops / (avg_ms * 1e-3) / 1e12
     roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[dom], "unit": "TFLOP/s",
             "frac": achieved / PEAK_TFLOPS[dom], "traffic": None,
             "kernel": KERNEL_SYMBOL[dom], "avg_launch_ms": avg_ms,
-            "flops_per_cell": FLOPS_PER_CELL[dom](dims), "cells_per_launch": st["cells"] / st["launches"]}
+            "flops_per_cell": FLOPS_PER_CELL[dom](dims),
+            "cells_per_launch": st["cells"] / st["launches"],
+            "evaluated_cells_per_launch": st["evaluated"] / st["launches"]}
 
     out = {
         "metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1, "steps": args.steps,

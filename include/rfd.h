@@ -67,7 +67,9 @@ typedef struct {
 typedef struct {
     int64_t launches[8];   /* index: rfd_kernel_kind */
     double ms[8];
-    double cells[8];       /* grid cells swept */
+    double cells[8];       /* grid cells swept (the algorithmic count) */
+    double evaluated[8];   /* cells whose weights / free intervals were computed;
+                              the rest were skipped as unable to change the result */
 } rfd_stats;
 
 typedef enum {

@@ -44,11 +44,12 @@ class Result(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [("launches", ctypes.c_int64 * 8), ("ms", ctypes.c_double * 8),
-                ("cells", ctypes.c_double * 8)]
+                ("cells", ctypes.c_double * 8), ("evaluated", ctypes.c_double * 8)]
 
     def as_dict(self) -> dict:
         return {KERNEL_NAMES[k]: dict(launches=int(self.launches[k]), ms=float(self.ms[k]),
-                                      cells=float(self.cells[k]))
+                                      cells=float(self.cells[k]),
+                                      evaluated=float(self.evaluated[k]))
                 for k in range(8) if self.launches[k]}
 
 

@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 struct rfd_ctx {
     int device = 0;
@@ -24,6 +25,10 @@ struct rfd_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int cu_count = 0;
     int max_waves = 0;
+    int sync = rfd::SYNC_SCOPED;  // band hand-off protocol (RFD_SYNC=0 selects acquire/release)
+    int rows_per_lane = 2;        // R of band_sweep (RFD_ROWS overrides: 1, 2 or 4)
+    bool prune = true;            // VE-FD skips cells above DFD^2 (RFD_PRUNE=0 disables)
+    const char *tlog_path = nullptr;  // RFD_TLOG: append per-band timestamps of every sweep
     rfd_stats stats{};
     char err[512] = {0};
 };
@@ -39,8 +44,10 @@ struct rfd_session {
     void *slots = nullptr;
     int *progress = nullptr;
     int *queue = nullptr;
-    double *result = nullptr;
+    double *result = nullptr;            // [0] terminal value, [1] its square (DFD, VE-FD)
+    unsigned long long *evals = nullptr; // cells evaluated by the last sweep
     double endpoint_lb = 0.0;
+    double dfd_sq = -1.0;                // DFD^2 of this pair once known (prunes VE-FD)
 };
 
 namespace {
@@ -87,12 +94,29 @@ int grid_for(rfd_ctx *ctx, int64_t nb) {
     return (int)std::max<int64_t>(g, 1);
 }
 
-template <int OP, typename T>
-hipError_t launch_sweep(int dims, int grid, hipStream_t st, const rfd::SweepArgs &a) {
+template <int OP, typename T, int SYNC, int R>
+void launch_sweep_r(int dims, int grid, hipStream_t st, const rfd::SweepArgs &a) {
     if (dims == 2)
-        hipLaunchKernelGGL((rfd::band_sweep<OP, 2, T>), dim3(grid), dim3(64), 0, st, a);
+        hipLaunchKernelGGL((rfd::band_sweep<OP, 2, T, SYNC, R>), dim3(grid), dim3(64), 0, st, a);
     else
-        hipLaunchKernelGGL((rfd::band_sweep<OP, 3, T>), dim3(grid), dim3(64), 0, st, a);
+        hipLaunchKernelGGL((rfd::band_sweep<OP, 3, T, SYNC, R>), dim3(grid), dim3(64), 0, st, a);
+}
+
+// Rows per lane the kernels are instantiated for.
+bool valid_rows_per_lane(int r) { return r == 1 || r == 2 || r == 4; }
+
+template <int OP, typename T>
+hipError_t launch_sweep(int dims, int grid, int sync, int rpl, hipStream_t st,
+                        const rfd::SweepArgs &a) {
+    if (sync == rfd::SYNC_ACQREL) {
+        launch_sweep_r<OP, T, rfd::SYNC_ACQREL, 1>(dims, grid, st, a);
+    } else if (rpl == 4) {
+        launch_sweep_r<OP, T, rfd::SYNC_SCOPED, 4>(dims, grid, st, a);
+    } else if (rpl == 2) {
+        launch_sweep_r<OP, T, rfd::SYNC_SCOPED, 2>(dims, grid, st, a);
+    } else {
+        launch_sweep_r<OP, T, rfd::SYNC_SCOPED, 1>(dims, grid, st, a);
+    }
     return hipGetLastError();
 }
 
@@ -129,11 +153,13 @@ rfd_status ensure_qseg(rfd_session *s, T **out, const T *Qd) {
 // context's stream, result copied back.
 template <int OP, typename T>
 rfd_status run_sweep(rfd_session *s, const T *P, const T *Q, const T *qseg, double param,
-                     rfd_kernel_kind kind, double *result) {
+                     rfd_kernel_kind kind, double *result, double thr = INFINITY,
+                     double *result_sq = nullptr) {
     rfd_ctx *ctx = s->ctx;
     const int64_t rows = (OP == rfd::OP_DFD) ? s->m : s->m - 1;
     const int64_t cols = (OP == rfd::OP_DFD) ? s->n : s->n - 1;
-    const int64_t nb = nbands_for(rows);
+    const int rpl = ctx->sync == rfd::SYNC_ACQREL ? 1 : ctx->rows_per_lane;
+    const int64_t nb = (rows + 64 * rpl - 1) / (64 * rpl);
     rfd::SweepArgs a{};
     a.P = P;
     a.Q = Q;
@@ -146,22 +172,49 @@ rfd_status run_sweep(rfd_session *s, const T *P, const T *Q, const T *qseg, doub
     a.cols = cols;
     a.nbands = (int)nb;
     a.param = param;
+    a.thr = thr;
+    a.evals = s->evals;
     const double sentinel = -3.0;  // .cu:193 (result = -3 until the terminal cell writes it)
+    unsigned long long *tlog = nullptr;
+    if (ctx->tlog_path) {
+        RFD_HIP(ctx, hipMalloc(&tlog, sizeof(unsigned long long) * 4 * nb));
+        RFD_HIP(ctx, hipMemsetAsync(tlog, 0, sizeof(unsigned long long) * 4 * nb, ctx->stream));
+    }
+    a.tlog = tlog;
+    const int grid = grid_for(ctx, nb);
     RFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     hipLaunchKernelGGL(rfd::sweep_init, dim3((unsigned)std::min<int64_t>((nb + 255) / 256, 1024)),
                        dim3(256), 0, ctx->stream, s->progress, (int)nb, s->queue, s->result,
-                       sentinel);
+                       sentinel, s->evals);
     RFD_HIP(ctx, hipGetLastError());
-    RFD_HIP(ctx, (launch_sweep<OP, T>(s->dims, grid_for(ctx, nb), ctx->stream, a)));
+    RFD_HIP(ctx, (launch_sweep<OP, T>(s->dims, grid, ctx->sync, rpl, ctx->stream, a)));
     RFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    double h = 0;
-    RFD_HIP(ctx, hipMemcpyAsync(&h, s->result, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    double hres[2] = {0, 0};
+    unsigned long long hev = 0;
+    RFD_HIP(ctx, hipMemcpyAsync(hres, s->result, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RFD_HIP(ctx, hipMemcpyAsync(&hev, s->evals, sizeof(hev), hipMemcpyDeviceToHost, ctx->stream));
     RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const double h = hres[0];
+    if (result_sq) *result_sq = hres[1];
     float ms = 0;
     RFD_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->stats.launches[kind] += 1;
     ctx->stats.ms[kind] += ms;
     ctx->stats.cells[kind] += (double)rows * (double)cols;
+    ctx->stats.evaluated[kind] += (double)hev;
+    if (tlog) {
+        // Debug instrumentation: header {kind, rows, cols, nbands, grid, sync} + 4 u64 per band.
+        std::vector<unsigned long long> h_t((size_t)(4 * nb));
+        RFD_HIP(ctx, hipMemcpy(h_t.data(), tlog, sizeof(unsigned long long) * 4 * nb, hipMemcpyDeviceToHost));
+        RFD_HIP(ctx, hipFree(tlog));
+        if (FILE *f = std::fopen(ctx->tlog_path, "ab")) {
+            long long hdr[6] = {(long long)kind, (long long)rows, (long long)cols, (long long)nb,
+                                (long long)grid, (long long)(ctx->sync + 10 * rpl)};
+            std::fwrite(hdr, sizeof(hdr), 1, f);
+            std::fwrite(h_t.data(), sizeof(unsigned long long), h_t.size(), f);
+            std::fclose(f);
+        }
+    }
     if (h == sentinel) {
         set_err(ctx, "sweep kernel %d wrote no terminal value", (int)kind);
         return RFD_ERR_STATE;
@@ -228,6 +281,13 @@ rfd_status rfd_create(rfd_ctx **out, int device) {
     ctx->max_waves = ctx->cu_count * waves_per_cu;
     // Absolute cap, used by tests to make waves take several bands each.
     if (const char *e = std::getenv("RFD_MAX_WAVES")) ctx->max_waves = std::max(1, std::atoi(e));
+    if (const char *e = std::getenv("RFD_SYNC")) ctx->sync = std::atoi(e) == 0 ? rfd::SYNC_ACQREL : rfd::SYNC_SCOPED;
+    ctx->tlog_path = std::getenv("RFD_TLOG");
+    if (const char *e = std::getenv("RFD_PRUNE")) ctx->prune = std::atoi(e) != 0;
+    if (const char *e = std::getenv("RFD_ROWS")) {
+        int r = std::atoi(e);
+        if (valid_rows_per_lane(r)) ctx->rows_per_lane = r;
+    }
     *out = ctx;
     return RFD_OK;
 }
@@ -268,6 +328,7 @@ rfd_status rfd_session_destroy(rfd_session *s) {
     (void)hipFree(s->progress);
     (void)hipFree(s->queue);
     (void)hipFree(s->result);
+    (void)hipFree(s->evals);
     delete s;
     return RFD_OK;
 }
@@ -305,7 +366,8 @@ rfd_status rfd_session_create(rfd_ctx *ctx, const double *P, int64_t m, const do
     if ((e = hipMalloc(&s->progress, sizeof(int) * (nbands_for(m) + 1))) != hipSuccess)
         return fail(e, "hipMalloc(progress)");
     if ((e = hipMalloc(&s->queue, sizeof(int))) != hipSuccess) return fail(e, "hipMalloc(queue)");
-    if ((e = hipMalloc(&s->result, sizeof(double))) != hipSuccess) return fail(e, "hipMalloc(result)");
+    if ((e = hipMalloc(&s->result, 2 * sizeof(double))) != hipSuccess) return fail(e, "hipMalloc(result)");
+    if ((e = hipMalloc(&s->evals, sizeof(unsigned long long))) != hipSuccess) return fail(e, "hipMalloc(evals)");
     if ((e = hipMemcpyAsync(s->dP, P, sizeof(double) * m * dims, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
         return fail(e, "hipMemcpy(P)");
     if ((e = hipMemcpyAsync(s->dQ, Q, sizeof(double) * n * dims, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
@@ -319,17 +381,36 @@ rfd_status rfd_session_bounds(rfd_session *s, double *dfd, double *vefd, int run
     if (!s || !dfd || !vefd) return RFD_ERR_ARG;
     rfd_ctx *ctx = s->ctx;
     RFD_HIP(ctx, hipSetDevice(ctx->device));
-    rfd_status st = run_sweep<rfd::OP_DFD, double>(s, s->dP, s->dQ, nullptr, 0.0, RFD_K_DFD, dfd);
+    double F = 0;
+    rfd_status st = run_sweep<rfd::OP_DFD, double>(s, s->dP, s->dQ, nullptr, 0.0, RFD_K_DFD, dfd,
+                                                   INFINITY, &F);
     if (st != RFD_OK) return st;
+    s->dfd_sq = F;
     if (run_vefd && s->m >= 2 && s->n >= 2) {
-        double init = *vefd;
-        if (fp32) {
-            if ((st = ensure_f32(s)) != RFD_OK) return st;
-            st = run_sweep<rfd::OP_VEFD, float>(s, s->fP, s->fQ, nullptr, init, RFD_K_VEFD32, vefd);
-        } else {
-            st = run_sweep<rfd::OP_VEFD, double>(s, s->dP, s->dQ, nullptr, init, RFD_K_VEFD64, vefd);
+        const double init = *vefd;
+        // VE-FD <= DFD: cells whose incoming bottleneck is at least DFD^2 (with a
+        // margin far above rounding) cannot change the minimum and are skipped.
+        // If the result ever reaches the threshold the sweep is rerun unpruned.
+        const bool prune = ctx->prune;
+        const double thr64 = prune ? F * (1.0 + 0x1p-40) : INFINITY;
+        const double thr32 = prune ? (double)(float)(F * (1.0 + 0x1p-18)) : INFINITY;
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            double sq = 0;
+            if (fp32) {
+                if ((st = ensure_f32(s)) != RFD_OK) return st;
+                const double thr = attempt ? INFINITY : thr32;
+                st = run_sweep<rfd::OP_VEFD, float>(s, s->fP, s->fQ, nullptr, init, RFD_K_VEFD32, vefd,
+                                                    thr, &sq);
+                if (st != RFD_OK) return st;
+                if (sq < thr) break;
+            } else {
+                const double thr = attempt ? INFINITY : thr64;
+                st = run_sweep<rfd::OP_VEFD, double>(s, s->dP, s->dQ, nullptr, init, RFD_K_VEFD64, vefd,
+                                                     thr, &sq);
+                if (st != RFD_OK) return st;
+                if (sq < thr) break;
+            }
         }
-        if (st != RFD_OK) return st;
     }
     return RFD_OK;
 }

@@ -59,9 +59,17 @@ __device__ inline float lane_read(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// Value of lane l-1 (lane 0 receives its own value; the caller overrides it).
-__device__ inline double shift_up(double v) { return __shfl_up(v, 1, 64); }
-__device__ inline float shift_up(float v) { return __shfl_up(v, 1, 64); }
+// Value of lane l-1, by DPP wave_shr:1 (one VALU move per dword, no LDS
+// round trip).  Lane 0 has no source lane and keeps `v`; the caller overrides it.
+__device__ inline int dpp_shr1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+__device__ inline double shift_up(double v) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    int lo = dpp_shr1((int)(unsigned)(b & 0xffffffffull));
+    int hi = dpp_shr1((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                            (unsigned long long)(unsigned)lo));
+}
+__device__ inline float shift_up(float v) { return __int_as_float(dpp_shr1(__float_as_int(v))); }
 
 // ---- per-cell formulas (identical op order to the CPU oracle) ----
 
@@ -152,20 +160,96 @@ struct SweepArgs {
     long long rows, cols;
     int nbands;
     double param;        // VE-FD: endpoint bound; decision: radius
+    double thr;          // VE-FD: prune threshold on incoming values (+inf: none)
+    unsigned long long *evals;  // cells whose weights / free intervals were evaluated
+    unsigned long long *tlog;   // optional, 4 per band: wave start, first input ready, end, hw id
 };
 
+// "Dead" values: those that can no longer change the sweep's result.
+//  VE-FD: an incoming bottleneck >= thr, where thr is slightly above DFD^2 >= VE-FD^2;
+//    a path through such a cell costs more than the optimum, so the cell's weights
+//    are not needed and it passes on its incoming value (itself dead).
+//  decision: a negative lower end (unreachable); such a cell's outputs are
+//    (-1, -2, -1), exactly what the full formula gives.
+//  DFD: nothing is dead.
+template <int OP, typename T>
+__device__ inline bool is_dead(T v, T thr) {
+    if (OP == OP_VEFD) return v >= thr;
+    if (OP == OP_DEC) return v < T(0);
+    return false;
+}
+
+// Band hand-off protocol.
+//  SYNC_ACQREL: agent-scope release store / acquire load of the progress flag;
+//    boundary values are plain loads/stores.  The compiler lowers these to an L2
+//    write-back (buffer_wbl2) before the flag store and an L2 invalidate
+//    (buffer_inv) after each flag load that succeeds or fails.
+//  SYNC_SCOPED: every access to the flag and to the boundary columns is an
+//    agent-scope atomic (sc1: coherent at device scope, no whole-cache
+//    maintenance).  The producer waits for its column stores to complete
+//    (s_waitcnt vmcnt(0)) before it stores the flag; the consumer issues its
+//    column loads only after the flag load has returned the needed count.
+enum SyncMode { SYNC_ACQREL = 0, SYNC_SCOPED = 1 };
+
+template <int SYNC>
+__device__ inline int flag_load(const int *p) {
+    if (SYNC == SYNC_ACQREL) return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <int SYNC, typename T>
+__device__ inline T column_load(const T *p) {
+    if (SYNC == SYNC_ACQREL) return *p;
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <int SYNC, typename T>
+__device__ inline void column_store(T *p, T v) {
+    if (SYNC == SYNC_ACQREL) *p = v;
+    else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <int SYNC>
+__device__ inline void flag_store(int *p, int v) {
+    if (SYNC == SYNC_ACQREL) {
+        __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __builtin_amdgcn_s_waitcnt(0);  // column stores of every lane have completed
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// One pair's sweep as the band code sees it.
+struct SweepJob {
+    const void *P, *Q, *qseg;  // T arrays (see SweepArgs)
+    void *slots;               // 2 * cols values of T
+    int *progress;             // per band of this pair
+    double *result;            // [0] terminal value, [1] its square (DFD, VE-FD)
+    long long rows, cols;
+    int nbands;
+    double param, thr;
+};
+
+template <int OP> struct Staged { static constexpr int value = (OP == OP_DEC) ? 2 : 1; };
+
+// Rows per lane.  Lane l of band b owns the R consecutive rows
+// 64*R*b + R*l + t, t = 0..R-1.  At step s it evaluates column s - l for all
+// R rows, top to bottom: row t's output feeds row t+1 in a register, and only
+// the last row's output crosses to lane l+1 (one shift per step for R cells).
+// The R weight evaluations of a step are independent, which gives each wave
+// instruction-level parallelism; the min/max chain is the only serial part.
+//
 // Q points (and, for the decision, the per-column terms) reach the lanes
 // through a ring of four 64-point chunks in LDS, structure-of-arrays so that
 // consecutive lanes read consecutive addresses.  Chunk k+2 is loaded into
 // registers at step 64k and written to LDS at step 64(k+1), so global-memory
-// latency stays off the per-step path.  At step s lane l reads point s-l+1
-// (s-l for DFD): chunks k-1, k, k+1 are live, the fourth slot is free.
-template <int OP, int D, typename T>
-__global__ __launch_bounds__(64) void band_sweep(SweepArgs a) {
+// latency stays off the per-step path.  At step s lane l reads points s-l and
+// s-l+1: chunks k-1, k, k+1 are live, the fourth slot is free.
+template <int OP, int D, typename T, int SYNC, int R>
+__device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4][D][64],
+                                T (&sring)[4][Staged<OP>::value][64], unsigned long long &evals,
+                                unsigned long long *tlog) {
     using N = Num<T>;
     constexpr int NS = (OP == OP_DEC) ? 2 : 0;  // per-column terms staged
-    __shared__ T qring[4][D][64];
-    __shared__ T sring[4][NS > 0 ? NS : 1][64];
+    constexpr int NP = (OP == OP_DFD) ? R : R + 1;  // P points a lane holds
     const int lane = (int)threadIdx.x;
     const T *P = (const T *)a.P;
     const T *Q = (const T *)a.Q;
@@ -174,136 +258,194 @@ __global__ __launch_bounds__(64) void band_sweep(SweepArgs a) {
     const long long rows = a.rows, cols = a.cols;
     // Points the sweep reads: DFD reads q_j (j < cols = n), the others q_j, q_{j+1}.
     const long long npts = (OP == OP_DFD) ? cols : cols + 1;
+    const long long prow_max = (OP == OP_DFD) ? rows - 1 : rows;  // last P index read
     const T big = N::big();
     const T par = (T)a.param;
     const T r2 = par * par;
+    const T thr = (T)a.thr;
+    const T dead_out = (OP == OP_DEC) ? T(-1) : big;  // value a dead cell passes on
 
-    for (;;) {
-        int band = 0;
-        if (lane == 0) band = atomicAdd(a.queue, 1);
-        band = __builtin_amdgcn_readfirstlane(band);
-        if (band >= a.nbands) break;
+    if (tlog && lane == 0) tlog[4 * band] = wall_clock64();
 
-        const long long i = (long long)band * 64 + lane;
-        const bool row_ok = i < rows;
-        const long long ii = row_ok ? i : rows - 1;
-        T pa[D], pb[D];
+    const long long i0 = (long long)band * 64 * R + (long long)R * lane;  // first row
+    T p[NP][D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-            pa[k] = P[ii * D + k];
-            pb[k] = (OP == OP_DFD) ? T(0) : P[(ii + 1) * D + k];
-        }
-        T A4a = T(0), inv2a = T(0);
+    for (int t = 0; t < NP; ++t) {
+        long long pi = i0 + t;
+        pi = pi < prow_max ? pi : prow_max;
+#pragma unroll
+        for (int k = 0; k < D; ++k) p[t][k] = P[pi * D + k];
+    }
+    T A4a[R], inv2a[R], bot[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        A4a[t] = T(0);
+        inv2a[t] = T(0);
         if (OP == OP_DEC) {
-            T Aa = seg_len2<T, D>(pa, pb);
-            A4a = T(4) * Aa;
-            inv2a = T(0.5) / Aa;
+            T Aa = seg_len2<T, D>(p[t], p[t + (OP == OP_DFD ? 0 : 1)]);
+            A4a[t] = T(4) * Aa;
+            inv2a[t] = T(0.5) / Aa;
         }
-        T bot;
-        if (OP == OP_DFD) bot = big;
-        else if (OP == OP_VEFD) bot = (i == 0) ? par * par : big;
-        else bot = (i == 0) ? T(0) : T(-1);
-        T out = (OP == OP_DEC) ? T(-1) : big;
+        const long long it = i0 + t;
+        if (OP == OP_DFD) bot[t] = big;
+        else if (OP == OP_VEFD) bot[t] = (it == 0) ? par * par : big;
+        else bot[t] = (it == 0) ? T(0) : T(-1);
+    }
+    T out = (OP == OP_DEC) ? T(-1) : big;
 
-        // Q staging: chunk 0 straight into LDS, chunk 1 into registers.
-        T pre[D], pres[NS > 0 ? NS : 1];
-        {
-            const long long p0 = lane;
+    // Q staging: chunk 0 straight into LDS, chunk 1 into registers.
+    T pre[D], pres[NS > 0 ? NS : 1];
+    {
+        const long long p0 = lane < npts ? lane : npts - 1;
 #pragma unroll
-            for (int k = 0; k < D; ++k) qring[0][k][lane] = (p0 < npts) ? Q[p0 * D + k] : T(0);
+        for (int k = 0; k < D; ++k) qring[0][k][lane] = Q[p0 * D + k];
+        if (NS > 0) {
+            const long long s0 = lane < cols ? lane : cols - 1;
+            sring[0][0][lane] = qseg[2 * s0];
+            sring[0][NS - 1][lane] = qseg[2 * s0 + 1];
+        }
+        long long p1 = 64 + lane;
+        p1 = p1 < npts ? p1 : npts - 1;
+#pragma unroll
+        for (int k = 0; k < D; ++k) pre[k] = Q[p1 * D + k];
+        if (NS > 0) {
+            const long long s1 = p1 < cols ? p1 : cols - 1;
+            pres[0] = qseg[2 * s1];
+            pres[NS - 1] = qseg[2 * s1 + 1];
+        }
+    }
+    const bool has_pred = band > 0;
+    const bool has_succ = band + 1 < a.nbands;
+    const T *sin = slots + (long long)((band + 1) & 1) * cols;
+    T *sout = slots + (long long)(band & 1) * cols;
+    T bin = T(0), bout = T(0);
+    const long long nsteps = cols + 63;
+
+    for (long long s = 0; s < nsteps; ++s) {
+        const long long j = s - lane;
+        if ((s & 63) == 0) {
+            // Stage chunk k+1 (in registers) into LDS.
+            const long long kc = s >> 6;
+            const int slot = (int)((kc + 1) & 3);
+#pragma unroll
+            for (int k = 0; k < D; ++k) qring[slot][k][lane] = pre[k];
             if (NS > 0) {
-                sring[0][0][lane] = (p0 < cols) ? qseg[2 * p0] : T(0);
-                sring[0][NS - 1][lane] = (p0 < cols) ? qseg[2 * p0 + 1] : T(0);
+                sring[slot][0][lane] = pres[0];
+                sring[slot][NS - 1][lane] = pres[NS - 1];
             }
-            const long long p1 = 64 + lane;
+            // Boundary values of this chunk; loaded before the prefetch below so
+            // that waiting for them does not wait for the prefetch.
+            if (has_pred && s < cols) {
+                const long long need = (s + 64 < cols) ? s + 64 : cols;
+                while (flag_load<SYNC>(&a.progress[band - 1]) < (int)need)
+                    __builtin_amdgcn_s_sleep(1);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                // Clamped (not predicated) so the load count is fixed and the
+                // wait for `bin` does not cover the prefetch issued after it.
+                const long long c = (s + lane < cols) ? s + lane : cols - 1;
+                bin = column_load<SYNC>(sin + c);
+                if (tlog && lane == 0 && s == 0) tlog[4 * band + 1] = wall_clock64();
+            }
+            // Dead chunk: every lane's state and in-flight value is dead and so is
+            // the boundary input for columns s..s+63.  Then every cell of steps
+            // s..s+63 is dead; skip them, publishing dead outputs for columns
+            // s-63..s (lane 63's outputs of those steps).
+            bool skip = false;
+            if (OP != OP_DFD && s >= 64 && s + 128 < cols) {
+                bool mine = is_dead<OP>(out, thr);
 #pragma unroll
-            for (int k = 0; k < D; ++k) pre[k] = (p1 < npts) ? Q[p1 * D + k] : T(0);
+                for (int t = 0; t < R; ++t) mine = mine && is_dead<OP>(bot[t], thr);
+                const T edge = has_pred ? bin : dead_out;
+                mine = mine && is_dead<OP>(edge, thr);
+                skip = __builtin_amdgcn_read_exec() == __ballot(mine);
+            }
+            // Fetch chunk k+2 into registers; it is written to LDS 64 steps later.
+            long long pn = (kc + 2) * 64 + lane;
+            pn = pn < npts ? pn : npts - 1;
+#pragma unroll
+            for (int k = 0; k < D; ++k) pre[k] = Q[pn * D + k];
             if (NS > 0) {
-                pres[0] = (p1 < cols) ? qseg[2 * p1] : T(0);
-                pres[NS - 1] = (p1 < cols) ? qseg[2 * p1 + 1] : T(0);
+                const long long ps = pn < cols ? pn : cols - 1;
+                pres[0] = qseg[2 * ps];
+                pres[NS - 1] = qseg[2 * ps + 1];
+            }
+            if (skip) {
+                if (has_succ) {
+                    // Columns s-64 (already in lane 0) .. s-1 form a full chunk.
+                    if (lane >= 1) bout = dead_out;
+                    column_store<SYNC>(sout + (s - 64) + lane, bout);
+                    __syncthreads();
+                    if (lane == 0) flag_store<SYNC>(&a.progress[band], (int)s);
+                    bout = dead_out;  // column s
+                }
+                s += 63;
+                continue;
             }
         }
-        // q_j of the lane's current column (VE-FD / decision); every lane starts at j = 0.
-        T qa[D];
-#pragma unroll
-        for (int k = 0; k < D; ++k) qa[k] = Q[k];
+        T bval;
+        if (has_pred) bval = lane_read(bin, (int)(s & 63));
+        else if (OP == OP_DFD) bval = (s == 0) ? T(0) : big;
+        else if (OP == OP_VEFD) bval = big;
+        else bval = T(-1);
+        T left = shift_up(out);
+        if (lane == 0) left = bval;
 
-        const bool has_pred = band > 0;
-        const bool has_succ = band + 1 < a.nbands;
-        const T *sin = slots + (long long)((band + 1) & 1) * cols;
-        T *sout = slots + (long long)(band & 1) * cols;
-        T bin = T(0), bout = T(0);
-        const long long nsteps = cols + 63;
-
-        for (long long s = 0; s < nsteps; ++s) {
-            const long long j = s - lane;
-            if ((s & 63) == 0) {
-                // Stage chunk k+1 (in registers) and fetch chunk k+2.
-                const long long kc = s >> 6;
-                const int slot = (int)((kc + 1) & 3);
+        if (j >= 0 && j < cols && i0 < rows) {
+            // q_j (and q_{j+1}) of this lane's column, from the LDS ring.
+            const int ra = (int)((j >> 6) & 3), oa = (int)(j & 63);
+            const long long jb = j + 1;
+            const int rb = (int)((jb >> 6) & 3), ob = (int)(jb & 63);
+            T qa[D], qb[D];
 #pragma unroll
-                for (int k = 0; k < D; ++k) qring[slot][k][lane] = pre[k];
-                if (NS > 0) {
-                    sring[slot][0][lane] = pres[0];
-                    sring[slot][NS - 1][lane] = pres[NS - 1];
-                }
-                const long long pn = (kc + 2) * 64 + lane;
-#pragma unroll
-                for (int k = 0; k < D; ++k) pre[k] = (pn < npts) ? Q[pn * D + k] : T(0);
-                if (NS > 0) {
-                    pres[0] = (pn < cols) ? qseg[2 * pn] : T(0);
-                    pres[NS - 1] = (pn < cols) ? qseg[2 * pn + 1] : T(0);
-                }
-                if (has_pred && s < cols) {
-                    const long long need = (s + 64 < cols) ? s + 64 : cols;
-                    while (__hip_atomic_load(&a.progress[band - 1], __ATOMIC_ACQUIRE,
-                                             __HIP_MEMORY_SCOPE_AGENT) < (int)need)
-                        __builtin_amdgcn_s_sleep(1);
-                    const long long c = s + lane;
-                    bin = (c < cols) ? sin[c] : T(0);
-                }
+            for (int k = 0; k < D; ++k) {
+                qa[k] = qring[ra][k][oa];
+                qb[k] = (OP == OP_DFD) ? qa[k] : qring[rb][k][ob];
             }
-            T bval;
-            if (has_pred) bval = lane_read(bin, (int)(s & 63));
-            else if (OP == OP_DFD) bval = (s == 0) ? T(0) : big;
-            else if (OP == OP_VEFD) bval = big;
-            else bval = T(-1);
-            T left = shift_up(out);
-            if (lane == 0) left = bval;
-
-            if (row_ok && j >= 0 && j < cols) {
-                const bool last = (i == rows - 1) && (j == cols - 1);
-                // LDS index of the point this lane reads this step.
-                const long long pt = (OP == OP_DFD) ? j : j + 1;
-                const int rs = (int)((pt >> 6) & 3), ro = (int)(pt & 63);
-                T qb[D];
+            T A4b = T(0), inv2b = T(0);
+            if (OP == OP_DEC) {
+                const int ss = (int)((j >> 6) & 3), so = (int)(j & 63);
+                A4b = sring[ss][0][so];
+                inv2b = sring[ss][NS - 1][so];
+            }
 #pragma unroll
-                for (int k = 0; k < D; ++k) qb[k] = qring[rs][k][ro];
+            for (int t = 0; t < R; ++t) {
+                const long long it = i0 + t;
+                if (R > 1 && it >= rows) break;
+                const bool last = (it == rows - 1) && (j == cols - 1);
                 if (OP == OP_DFD) {
-                    T d = dist2<T, D>(pa, qb);
-                    T in = N::min(left, bot);
+                    T d = dist2<T, D>(p[t], qa);
+                    T in = N::min(left, bot[t]);
                     T top = N::max(d, in);
-                    out = N::min(top, bot);
-                    bot = top;
-                    if (last) *a.result = (double)N::sqrt(top);
+                    left = N::min(top, bot[t]);
+                    bot[t] = top;
+                    if (last) {
+                        a.result[0] = (double)N::sqrt(top);
+                        a.result[1] = (double)top;
+                    }
                 } else if (OP == OP_VEFD) {
-                    T in = N::min(left, bot);
-                    T wr = point_seg2<T, D>(pb, qa, qb);
-                    T wt = point_seg2<T, D>(qb, pa, pb);
-                    T Tv = N::max(wt, in);
-                    T Rv = N::max(wr, in);
-                    bot = Tv;
-                    out = Rv;
-                    if (last) *a.result = (double)N::sqrt(N::min(Tv, Rv));
+                    T in = N::min(left, bot[t]);
+                    T Tv = in, Rv = in;
+                    if (in < thr) {
+                        T wr = point_seg2<T, D>(p[t + 1], qa, qb);
+                        T wt = point_seg2<T, D>(qb, p[t], p[t + 1]);
+                        Tv = N::max(wt, in);
+                        Rv = N::max(wr, in);
+                        ++evals;
+                    }
+                    bot[t] = Tv;
+                    left = Rv;
+                    if (last) {
+                        a.result[0] = (double)N::sqrt(N::min(Tv, Rv));
+                        a.result[1] = (double)N::min(Tv, Rv);
+                    }
                 } else {
-                    const T leftLB = left, botLB = bot;
+                    const T leftLB = left, botLB = bot[t];
                     T rLB = T(-1), rUB = T(-2), tLB = T(-1);
                     if (leftLB >= T(0) || botLB >= T(0)) {
-                        const int ss = (int)((j >> 6) & 3), so = (int)(j & 63);
-                        const T A4b = sring[ss][0][so], inv2b = sring[ss][NS - 1][so];
+                        ++evals;
                         T tUB;
-                        free_iv<T, D>(pb, qa, qb, r2, A4b, inv2b, rLB, rUB);
-                        free_iv<T, D>(qb, pa, pb, r2, A4a, inv2a, tLB, tUB);
+                        free_iv<T, D>(p[t + 1], qa, qb, r2, A4b, inv2b, rLB, rUB);
+                        free_iv<T, D>(qb, p[t], p[t + 1], r2, A4a[t], inv2a[t], tLB, tUB);
                         if (botLB < T(0)) {
                             bool inv = leftLB > rUB || leftLB < T(0);
                             T t2 = N::max(rLB, leftLB);
@@ -316,40 +458,62 @@ __global__ __launch_bounds__(64) void band_sweep(SweepArgs a) {
                             tLB = inv ? T(-1) : t2;
                         }
                     }
-                    bot = tLB;
-                    out = rLB;
+                    bot[t] = tLB;
+                    left = rLB;
                     if (last) *a.result = (double)rUB;
                 }
-                if (OP != OP_DFD) {
-#pragma unroll
-                    for (int k = 0; k < D; ++k) qa[k] = qb[k];
-                }
             }
+            out = left;
+        }
 
-            if (has_succ && s >= 63) {
-                const long long col = s - 63;
-                const T v = lane_read(out, 63);
-                if (lane == (int)(col & 63)) bout = v;
-                if ((col & 63) == 63 || col == cols - 1) {
-                    const long long base = col & ~63LL;
-                    if (base + lane <= col) sout[base + lane] = bout;
-                    __syncthreads();
-                    if (lane == 0)
-                        __hip_atomic_store(&a.progress[band], (int)(col + 1), __ATOMIC_RELEASE,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-                }
+        if (has_succ && s >= 63) {
+            const long long col = s - 63;
+            const T v = lane_read(out, 63);
+            if (lane == (int)(col & 63)) bout = v;
+            if ((col & 63) == 63 || col == cols - 1) {
+                const long long base = col & ~63LL;
+                if (base + lane <= col) column_store<SYNC>(sout + base + lane, bout);
+                __syncthreads();
+                if (lane == 0) flag_store<SYNC>(&a.progress[band], (int)(col + 1));
             }
         }
     }
+    if (OP == OP_DFD) evals += (unsigned long long)((i0 < rows) ? ((rows - i0 < R) ? rows - i0 : R) : 0) * cols;
+    if (tlog && lane == 0) {
+        tlog[4 * band + 2] = wall_clock64();
+        tlog[4 * band + 3] = ((unsigned long long)__builtin_amdgcn_s_getreg((20 << 0) | (0 << 6) | (31 << 11)) << 32) |
+                              (unsigned long long)__builtin_amdgcn_s_getreg((4 << 0) | (0 << 6) | (31 << 11));
+    }
 }
 
-__global__ void sweep_init(int *progress, int nbands, int *queue, double *result, double sentinel) {
+template <int OP, int D, typename T, int SYNC, int R>
+__global__ __launch_bounds__(64) void band_sweep(SweepArgs a) {
+    __shared__ T qring[4][D][64];
+    __shared__ T sring[4][Staged<OP>::value][64];
+    const int lane = (int)threadIdx.x;
+    SweepJob job{a.P, a.Q, a.qseg, a.slots, a.progress, a.result, a.rows, a.cols, a.nbands,
+                 a.param, a.thr};
+    unsigned long long evals = 0;
+    for (;;) {
+        int band = 0;
+        if (lane == 0) band = atomicAdd(a.queue, 1);
+        band = __builtin_amdgcn_readfirstlane(band);
+        if (band >= a.nbands) break;
+        run_band<OP, D, T, SYNC, R>(job, band, qring, sring, evals, a.tlog);
+    }
+    if (a.evals) atomicAdd(a.evals, evals);
+}
+
+__global__ void sweep_init(int *progress, int nbands, int *queue, double *result, double sentinel,
+                           unsigned long long *evals) {
     const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int stride = (int)(gridDim.x * blockDim.x);
     for (int k = tid; k < nbands; k += stride) progress[k] = 0;
     if (tid == 0) {
         *queue = 0;
-        *result = sentinel;
+        result[0] = sentinel;
+        result[1] = sentinel;
+        *evals = 0;
     }
 }
 

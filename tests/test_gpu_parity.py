@@ -26,7 +26,7 @@ def walk(rng, k, d, sigma=1.0):
 
 
 SHAPES = [(2, 2), (2, 3), (3, 2), (5, 7), (63, 40), (64, 64), (65, 66), (66, 65), (128, 70),
-          (129, 300), (200, 129), (700, 50), (40, 700)]
+          (129, 300), (200, 129), (700, 50), (40, 700), (257, 100), (520, 77)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -131,3 +131,28 @@ def test_mixed_precision_result_within_tolerance(gpu):
     got = distance(P, Q, stop="rel", eps=1e-9, precision="mixed", ctx=gpu)
     assert abs(got["distance"] - ref["distance"]) <= 1e-9 * ref["distance"]
     assert got["lb"] <= ref["ub"] and ref["lb"] <= got["ub"]
+
+
+@pytest.mark.parametrize("kind", ["walk2", "walk3", "robot"])
+def test_pruned_and_skipped_sweeps_bit_exact(gpu, kind):
+    """Sizes where VE-FD pruning (cells above DFD^2) and 64-step dead-chunk
+    skipping both occur; results must not change."""
+    if kind == "robot":
+        P, Q = W.robot_pair(1500, 1800, seed=3)
+    else:
+        rng = np.random.default_rng(17 if kind == "walk2" else 18)
+        d = 2 if kind == "walk2" else 3
+        P, Q = walk(rng, 1400, d), walk(rng, 1700, d)
+    say("pruned", kind)
+    lb = O.endpoint_lb(P, Q)
+    dfd = O.dfd(P, Q)
+    with Intersection(P, Q, ctx=gpu) as s:
+        gd, gv = s.Bound_call_gpu(lb)
+        assert gd == dfd
+        assert gv == O.vefd(P, Q, lb)
+        _, gv32 = s.Bound_call_gpu(lb, fp32=True)
+        assert gv32 == O.vefd(P, Q, lb, fp32=True)
+        st = gpu.stats()
+        assert st["vefd_f64"]["evaluated"] <= st["vefd_f64"]["cells"]
+        for r in (lb, 0.5 * (lb + gv), gv * (1 - 1e-9), gv, 0.5 * (gv + dfd), dfd):
+            assert s.decide_ub(r) == O.decide_ub(P, Q, r)
