@@ -48,6 +48,9 @@ typedef struct {
     int32_t vefd_gate;     /* 1: run VE-FD only when m*n > 10000 (.cu:1343); 0: always */
     int32_t verbose;       /* 1: print the reference's progress lines (frechet.cpp:21-71) */
     double mixed_switch;   /* MIXED: relative bracket width below which FP64 takes over */
+    int32_t rows_per_lane; /* sweep tile height per lane: 0 = library choice, else 1, 2 or 4 */
+    int32_t use_graph;     /* batch: 1 = replay the captured hipGraph of all rounds */
+    int32_t prune;         /* VE-FD cells above DFD^2 skipped: -1 = library default, 0, 1 */
 } rfd_opts;
 
 typedef struct {
@@ -121,6 +124,27 @@ rfd_status rfd_session_distance(rfd_session *s, const rfd_opts *opts, rfd_result
  * rfd_distance with stop_mode = RFD_STOP_ABS, eps = FLT_EPSILON. */
 rfd_status rfd_distance(rfd_ctx *ctx, const double *P, int64_t m, const double *Q, int64_t n,
                         int dims, const rfd_opts *opts, rfd_result *res);
+
+/* ---- Batches: many pairs, resident on the device, one launch per round ----
+ * No counterpart in the reference, which computes one pair per call.  All
+ * rounds (DFD, VE-FD, the bisection's decisions) of all pairs run as
+ * persistent kernels on the context's stream; each pair keeps its own bracket
+ * on the device, so no host round trip happens between rounds. */
+typedef struct rfd_batch rfd_batch;
+
+/* P[k], Q[k]: AoS curves of pair k with m[k], n[k] points of dims[k] in {2,3}. */
+rfd_status rfd_batch_create(rfd_ctx *ctx, int64_t npairs, const double *const *P,
+                            const int64_t *m, const double *const *Q, const int64_t *n,
+                            const int32_t *dims, rfd_batch **out);
+/* Distances of every pair (results[k], frechet.cpp semantics per pair).
+ * latency_ms (may be NULL): per pair, time from the batch start to the
+ * round that finished that pair. */
+rfd_status rfd_batch_run(rfd_batch *b, const rfd_opts *opts, rfd_result *results,
+                         double *latency_ms);
+/* Times the batch once for each rows-per-lane choice (1, 2, 4), keeps the
+ * fastest for later runs and returns it; ms (may be NULL) gets 3 timings. */
+rfd_status rfd_batch_tune(rfd_batch *b, const rfd_opts *opts, int32_t *rows_per_lane, double *ms);
+rfd_status rfd_batch_destroy(rfd_batch *b);
 
 #ifdef __cplusplus
 }

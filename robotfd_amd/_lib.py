@@ -27,7 +27,9 @@ KERNEL_NAMES = ["dfd", "vefd_f64", "decide_f64", "vefd_f32", "decide_f32", "batc
 class Opts(ctypes.Structure):
     _fields_ = [("stop_mode", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("eps", ctypes.c_double), ("vefd_gate", ctypes.c_int32),
-                ("verbose", ctypes.c_int32), ("mixed_switch", ctypes.c_double)]
+                ("verbose", ctypes.c_int32), ("mixed_switch", ctypes.c_double),
+                ("rows_per_lane", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+                ("prune", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):
@@ -74,6 +76,12 @@ SIGNATURES = {
     "rfd_session_distance": (ctypes.c_int, [_vp, ctypes.POINTER(Opts), ctypes.POINTER(Result)]),
     "rfd_distance": (ctypes.c_int, [_vp, _dp, _i64, _dp, _i64, ctypes.c_int,
                                     ctypes.POINTER(Opts), ctypes.POINTER(Result)]),
+    "rfd_batch_create": (ctypes.c_int, [_vp, _i64, ctypes.POINTER(_dp), ctypes.POINTER(_i64),
+                                        ctypes.POINTER(_dp), ctypes.POINTER(_i64),
+                                        ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(_vp)]),
+    "rfd_batch_run": (ctypes.c_int, [_vp, ctypes.POINTER(Opts), ctypes.POINTER(Result), _dp]),
+    "rfd_batch_tune": (ctypes.c_int, [_vp, ctypes.POINTER(Opts), ctypes.POINTER(ctypes.c_int32), _dp]),
+    "rfd_batch_destroy": (ctypes.c_int, [_vp]),
 }
 
 _LIB = None

@@ -1,0 +1,257 @@
+// Batched sweeps: many pairs of uneven length in one persistent launch per
+// round, with each pair's bisection state kept on the device.
+//
+// A round is: plan (one workgroup: lists the bands of every pair still active,
+// longest pairs first, and clears their progress flags) -> sweep (persistent
+// waves take (pair, band) items from the list in order and run the same band
+// code as the single-pair path) -> update (one thread per pair: consumes the
+// sweep's terminal value, moves the bracket, decides whether the pair stays
+// active).  All of it is enqueued on one stream and captured as a hipGraph.
+//
+// Curves are stored 3D: a 2D pair gets z = 0, which leaves every distance,
+// weight and free interval bit-identical to the 2D evaluation (each extra term
+// is fma(0, 0, acc) = acc).
+#pragma once
+
+#include "sweep_kernels.hpp"
+
+namespace rfd {
+
+enum BatchPhase { PH_DFD = 0, PH_VEFD = 1, PH_VEFD_REDO = 2, PH_DEC = 3 };
+
+struct BatchPair {
+    long long m, n;          // points
+    long long poff, qoff;    // offsets (in doubles) of P and Q in the point pool (3 per point)
+    long long qsoff;         // offset of the decision's per-column terms (2 per column)
+    long long sloff;         // offset of the two boundary columns (2 * n)
+    long long bandoff;       // offset of this pair's progress flags
+    double endpoint_lb;      // frechet.cpp:30, computed on the host
+    int big;                 // m*n > 10000 (.cu:1343, in 64 bits)
+    int pad;
+};
+
+struct BatchState {
+    double lb, ub, split, dfd, vefd, F, distance;
+    double result[2];        // terminal value of the pair's last sweep
+    double thr;              // VE-FD prune threshold used
+    long long iterations;
+    unsigned long long t_done;  // wall clock when the pair's result became final
+    int active;
+    int need_redo;
+};
+
+struct BatchArgs {
+    const BatchPair *pairs;
+    BatchState *state;
+    const int *order;        // pair indices, longest first
+    int npairs;
+    const double *pts;
+    const double *qseg;
+    double *slots;
+    int *progress;
+    int2 *items;             // (pair, band) work list of the current round
+    int *count;              // [0] items, [1] queue
+    unsigned long long *evals;
+    unsigned long long *t0;  // wall clock at batch start
+    int rows_per_lane;
+    int stop_mode;
+    double eps;
+    int prune;
+    int gate;                // 1: VE-FD only when m*n > 10000
+};
+
+__device__ inline bool batch_keep(double lb, double ub, int stop_mode, double eps) {
+    return stop_mode == 0 ? (ub - lb > eps) : (ub - lb > eps * ub);
+}
+
+__device__ inline long long batch_rows(const BatchPair &p, int phase) {
+    return phase == PH_DFD ? p.m : p.m - 1;
+}
+
+__device__ inline bool batch_runs_vefd(const BatchPair &p, int gate) {
+    return p.m >= 2 && p.n >= 2 && (!gate || p.big);
+}
+
+// Whether pair k takes part in a round of `phase`.
+__device__ inline bool batch_in_round(const BatchPair &p, const BatchState &st, int phase, int gate) {
+    if (phase == PH_DFD) return true;
+    if (p.m < 2 || p.n < 2) return false;
+    if (phase == PH_VEFD) return batch_runs_vefd(p, gate);
+    if (phase == PH_VEFD_REDO) return st.need_redo != 0;
+    return st.active != 0;
+}
+
+__global__ void batch_reset(BatchArgs a) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k == 0) *a.t0 = wall_clock64();
+    if (k >= a.npairs) return;
+    BatchState &st = a.state[k];
+    const BatchPair &p = a.pairs[k];
+    st.lb = p.endpoint_lb;
+    st.ub = 0;
+    st.split = 0;
+    st.dfd = 0;
+    st.vefd = p.endpoint_lb;
+    st.F = 0;
+    st.distance = 0;
+    st.result[0] = st.result[1] = -3.0;
+    st.thr = INFINITY;
+    st.iterations = 0;
+    st.t_done = 0;
+    st.active = 0;
+    st.need_redo = 0;
+    if (k == 0) *a.evals = 0;
+}
+
+// One workgroup of 1024 threads: exclusive scan of per-pair band counts in
+// `order`, then every thread writes its pairs' items and clears their flags.
+__global__ __launch_bounds__(1024) void batch_plan(BatchArgs a, int phase) {
+    __shared__ long long part[1024];
+    const int tid = (int)threadIdx.x;
+    const int per = (a.npairs + 1023) / 1024;
+    const int k0 = tid * per;
+    const int R = a.rows_per_lane;
+    long long mine = 0;
+    for (int q = 0; q < per; ++q) {
+        const int idx = k0 + q;
+        if (idx >= a.npairs) break;
+        const int k = a.order[idx];
+        const BatchPair &p = a.pairs[k];
+        if (batch_in_round(p, a.state[k], phase, a.gate)) {
+            const long long rows = batch_rows(p, phase);
+            mine += (rows + 64LL * R - 1) / (64LL * R);
+        }
+    }
+    part[tid] = mine;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        long long v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    long long pos = part[tid] - mine;
+    for (int q = 0; q < per; ++q) {
+        const int idx = k0 + q;
+        if (idx >= a.npairs) break;
+        const int k = a.order[idx];
+        const BatchPair &p = a.pairs[k];
+        if (!batch_in_round(p, a.state[k], phase, a.gate)) continue;
+        const long long rows = batch_rows(p, phase);
+        const int nb = (int)((rows + 64LL * R - 1) / (64LL * R));
+        for (int b = 0; b < nb; ++b) {
+            a.items[pos + b] = make_int2(k, b);
+            a.progress[p.bandoff + b] = 0;
+        }
+        pos += nb;
+        a.state[k].result[0] = -3.0;
+        a.state[k].result[1] = -3.0;
+        if (phase == PH_VEFD || phase == PH_VEFD_REDO) {
+            const double F = a.state[k].F;
+            a.state[k].thr = (phase == PH_VEFD && a.prune) ? F * (1.0 + 0x1p-40) : INFINITY;
+        }
+    }
+    if (tid == 1023) {
+        a.count[0] = (int)part[1023];
+        a.count[1] = 0;
+    }
+}
+
+template <int OP, int R>
+__global__ __launch_bounds__(64) void batch_sweep(BatchArgs a) {
+    __shared__ double qring[4][3][64];
+    __shared__ double sring[4][Staged<OP>::value][64];
+    const int lane = (int)threadIdx.x;
+    unsigned long long evals = 0;
+    const int total = __hip_atomic_load(&a.count[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        int idx = 0;
+        if (lane == 0) idx = atomicAdd(&a.count[1], 1);
+        idx = __builtin_amdgcn_readfirstlane(idx);
+        if (idx >= total) break;
+        const int2 it = a.items[idx];
+        const int k = __builtin_amdgcn_readfirstlane(it.x);
+        const int band = __builtin_amdgcn_readfirstlane(it.y);
+        const BatchPair p = a.pairs[k];
+        BatchState &st = a.state[k];
+        const long long rows = (OP == OP_DFD) ? p.m : p.m - 1;
+        const long long cols = (OP == OP_DFD) ? p.n : p.n - 1;
+        SweepJob job;
+        job.P = a.pts + p.poff;
+        job.Q = a.pts + p.qoff;
+        job.qseg = a.qseg + p.qsoff;
+        job.slots = a.slots + p.sloff;
+        job.progress = a.progress + p.bandoff;
+        job.result = st.result;
+        job.rows = rows;
+        job.cols = cols;
+        job.nbands = (int)((rows + 64LL * R - 1) / (64LL * R));
+        job.param = (OP == OP_DEC) ? st.split : (OP == OP_VEFD ? p.endpoint_lb : 0.0);
+        job.thr = (OP == OP_VEFD) ? st.thr : INFINITY;
+        run_band<OP, 3, double, SYNC_SCOPED, R>(job, band, qring, sring, evals, nullptr);
+    }
+    if (a.evals) atomicAdd(a.evals, evals);
+}
+
+// Decision per-column terms of every pair (4*Ab, 0.5/Ab), once per batch.
+__global__ void batch_qseg(BatchArgs a) {
+    const int k = (int)blockIdx.y;
+    if (k >= a.npairs) return;
+    const BatchPair p = a.pairs[k];
+    const double *Q = a.pts + p.qoff;
+    double *qs = const_cast<double *>(a.qseg) + p.qsoff;
+    for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < p.n - 1;
+         j += (long long)gridDim.x * blockDim.x) {
+        double Ab = seg_len2<double, 3>(Q + j * 3, Q + (j + 1) * 3);
+        qs[2 * j] = 4.0 * Ab;
+        qs[2 * j + 1] = 0.5 / Ab;
+    }
+}
+
+__global__ void batch_update(BatchArgs a, int phase) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= a.npairs) return;
+    const BatchPair &p = a.pairs[k];
+    BatchState &st = a.state[k];
+    if (!batch_in_round(p, st, phase, a.gate)) return;
+    if (phase == PH_DFD) {
+        st.dfd = st.result[0];
+        st.F = st.result[1];
+        st.ub = st.dfd;
+        st.lb = p.endpoint_lb;
+        if (p.m < 2 || p.n < 2) {
+            // One point against a curve: the DFD is the distance.
+            st.lb = st.ub;
+            st.distance = st.dfd;
+            st.active = 0;
+            st.t_done = wall_clock64();
+            return;
+        }
+        if (!batch_runs_vefd(p, a.gate)) {
+            st.active = batch_keep(st.lb, st.ub, a.stop_mode, a.eps) ? 1 : 0;
+            st.split = 0.5 * (st.ub + st.lb);
+            st.distance = (st.ub + st.lb) / 2.0;
+            if (!st.active) st.t_done = wall_clock64();
+        }
+        return;
+    }
+    if (phase == PH_VEFD || phase == PH_VEFD_REDO) {
+        if (phase == PH_VEFD && st.result[1] >= st.thr) {  // pruning reached: rerun unpruned
+            st.need_redo = 1;
+            return;
+        }
+        st.need_redo = 0;
+        st.vefd = st.result[0];
+        st.lb = st.vefd;
+    } else {
+        const bool reach = st.result[0] == 1.0;  // .cu:1592
+        if (reach) st.ub = st.split; else st.lb = st.split;
+        st.iterations += 1;
+    }
+    st.active = batch_keep(st.lb, st.ub, a.stop_mode, a.eps) ? 1 : 0;
+    st.split = (st.ub + st.lb) / 2.0;     // frechet.cpp:42
+    st.distance = (st.ub + st.lb) / 2.0;  // frechet.cpp:63
+    if (!st.active) st.t_done = wall_clock64();
+}
+
+}  // namespace rfd

@@ -5,6 +5,7 @@
 // kernels (sweep_kernels.hpp), and the bisection of frechet.cpp:18-75 runs on
 // the host between decision sweeps.
 #include "rfd.h"
+#include "rfd_internal.hpp"
 #include "sweep_kernels.hpp"
 
 #include <hip/hip_runtime.h>
@@ -18,20 +19,6 @@
 #include <cstring>
 #include <string>
 #include <vector>
-
-struct rfd_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int cu_count = 0;
-    int max_waves = 0;
-    int sync = rfd::SYNC_SCOPED;  // band hand-off protocol (RFD_SYNC=0 selects acquire/release)
-    int rows_per_lane = 2;        // R of band_sweep (RFD_ROWS overrides: 1, 2 or 4)
-    bool prune = true;            // VE-FD skips cells above DFD^2 (RFD_PRUNE=0 disables)
-    const char *tlog_path = nullptr;  // RFD_TLOG: append per-band timestamps of every sweep
-    rfd_stats stats{};
-    char err[512] = {0};
-};
 
 struct rfd_session {
     rfd_ctx *ctx = nullptr;
@@ -50,42 +37,32 @@ struct rfd_session {
     double dfd_sq = -1.0;                // DFD^2 of this pair once known (prunes VE-FD)
 };
 
+namespace rfd {
+
+__global__ void sweep_init(int *progress, int nbands, int *queue, double *result, double sentinel,
+                           unsigned long long *evals) {
+    const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int stride = (int)(gridDim.x * blockDim.x);
+    for (int k = tid; k < nbands; k += stride) progress[k] = 0;
+    if (tid == 0) {
+        *queue = 0;
+        result[0] = sentinel;
+        result[1] = sentinel;
+        *evals = 0;
+    }
+}
+
+__global__ void to_f32(const double *src, float *dst, long long count) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) dst[k] = (float)src[k];
+}
+
+}  // namespace rfd
+
 namespace {
 
-void set_err(rfd_ctx *ctx, const char *fmt, ...) {
-    if (!ctx) return;
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(ctx->err, sizeof(ctx->err), fmt, ap);
-    va_end(ap);
-}
-
-#define RFD_HIP(ctx, call)                                                          \
-    do {                                                                            \
-        hipError_t e_ = (call);                                                     \
-        if (e_ != hipSuccess) {                                                     \
-            set_err((ctx), "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
-                    __FILE__, __LINE__);                                            \
-            return RFD_ERR_HIP;                                                     \
-        }                                                                           \
-    } while (0)
-
-// frechet.cpp:30 with Point::dist_sqr (geometry_basics.hpp:72-81): host
-// arithmetic, each product rounded before the sum.
-double endpoint_bound(const double *P, int64_t m, const double *Q, int64_t n, int dims) {
-    auto d2 = [&](const double *a, const double *b) {
-        double result = 0.0;
-        for (int k = 0; k < dims; ++k) {
-            volatile double ds = a[k] - b[k];
-            volatile double sq = ds * ds;
-            result = result + sq;
-        }
-        return result;
-    };
-    double a = d2(P, Q);
-    double b = d2(P + (m - 1) * dims, Q + (n - 1) * dims);
-    return std::sqrt(std::max(a, b));
-}
+using rfd_host::set_err;
+using rfd_host::endpoint_bound;
 
 int64_t nbands_for(int64_t rows) { return (rows + 63) / 64; }
 
@@ -254,6 +231,9 @@ void rfd_default_opts(rfd_opts *o) {
     o->vefd_gate = 1;
     o->verbose = 0;
     o->mixed_switch = 1e-4;
+    o->rows_per_lane = 0;
+    o->use_graph = 1;
+    o->prune = -1;
 }
 
 rfd_status rfd_create(rfd_ctx **out, int device) {
@@ -276,6 +256,11 @@ rfd_status rfd_create(rfd_ctx **out, int device) {
         return RFD_ERR_HIP;
     }
     ctx->cu_count = prop.multiProcessorCount;
+    {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+            ctx->wall_khz = (double)khz;
+    }
     int waves_per_cu = 16;
     if (const char *e = std::getenv("RFD_WAVES_PER_CU")) waves_per_cu = std::max(1, std::atoi(e));
     ctx->max_waves = ctx->cu_count * waves_per_cu;

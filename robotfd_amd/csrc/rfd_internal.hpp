@@ -1,0 +1,70 @@
+// Host-side internals shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+
+#include "rfd.h"
+#include "sweep_kernels.hpp"
+
+struct rfd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int cu_count = 0;
+    int max_waves = 0;
+    double wall_khz = 100000.0;   // device wall clock (wall_clock64) rate
+    int sync = rfd::SYNC_SCOPED;  // band hand-off protocol (RFD_SYNC=0 selects acquire/release)
+    int rows_per_lane = 4;        // R of band_sweep (RFD_ROWS overrides: 1, 2 or 4)
+    bool prune = false;           // VE-FD skips cells above DFD^2 (RFD_PRUNE=1 enables)
+    const char *tlog_path = nullptr;  // RFD_TLOG: append per-band timestamps of every sweep
+    rfd_stats stats{};
+    char err[512] = {0};
+};
+
+namespace rfd_host {
+
+inline void set_err(rfd_ctx *ctx, const char *fmt, ...) {
+    if (!ctx) return;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(ctx->err, sizeof(ctx->err), fmt, ap);
+    va_end(ap);
+}
+
+}  // namespace rfd_host
+
+#define RFD_HIP(ctx, call)                                                          \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            rfd_host::set_err((ctx), "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
+                    __FILE__, __LINE__);                                            \
+            return RFD_ERR_HIP;                                                     \
+        }                                                                           \
+    } while (0)
+
+namespace rfd_host {
+
+// frechet.cpp:30 with Point::dist_sqr (geometry_basics.hpp:72-81): host
+// arithmetic, each product rounded before the sum.
+inline double endpoint_bound(const double *P, int64_t m, const double *Q, int64_t n, int dims) {
+    auto d2 = [&](const double *a, const double *b) {
+        double result = 0.0;
+        for (int k = 0; k < dims; ++k) {
+            volatile double ds = a[k] - b[k];
+            volatile double sq = ds * ds;
+            result = result + sq;
+        }
+        return result;
+    };
+    double a = d2(P, Q);
+    double b = d2(P + (m - 1) * dims, Q + (n - 1) * dims);
+    return std::sqrt(std::max(a, b));
+}
+
+}  // namespace rfd_host

@@ -504,19 +504,6 @@ __global__ __launch_bounds__(64) void band_sweep(SweepArgs a) {
     if (a.evals) atomicAdd(a.evals, evals);
 }
 
-__global__ void sweep_init(int *progress, int nbands, int *queue, double *result, double sentinel,
-                           unsigned long long *evals) {
-    const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int stride = (int)(gridDim.x * blockDim.x);
-    for (int k = tid; k < nbands; k += stride) progress[k] = 0;
-    if (tid == 0) {
-        *queue = 0;
-        result[0] = sentinel;
-        result[1] = sentinel;
-        *evals = 0;
-    }
-}
-
 // Per-column terms of the decision sweep (.cu:1129-1144): 4*Ab and 0.5/Ab.
 template <int D, typename T>
 __global__ void qseg_terms(const T *Q, long long cols, T *qseg) {
@@ -525,11 +512,6 @@ __global__ void qseg_terms(const T *Q, long long cols, T *qseg) {
     T Ab = seg_len2<T, D>(Q + j * D, Q + (j + 1) * D);
     qseg[2 * j] = T(4) * Ab;
     qseg[2 * j + 1] = T(0.5) / Ab;
-}
-
-__global__ void to_f32(const double *src, float *dst, long long count) {
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < count) dst[k] = (float)src[k];
 }
 
 }  // namespace rfd

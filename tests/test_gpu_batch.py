@@ -1,0 +1,92 @@
+"""Batched pipeline parity: every pair of a batch against the CPU oracle,
+bit-exact, for every tile height, with and without the hipGraph, and across
+graph replays."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from robotfd_amd import workloads as W
+from robotfd_amd.batch import Batch
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_pairs(count, seed, lo=60, hi=900):
+    rng = np.random.default_rng(seed)
+    pairs = []
+    for k in range(count):
+        m, n = (np.exp(rng.uniform(np.log(lo), np.log(hi), 2))).astype(int)
+        d = 2 if rng.uniform() < 0.5 else 3
+        if k % 3 == 0:
+            P, Q = W.robot_pair(max(m, 20), max(n, 20), seed=seed * 1000 + k)
+            if d == 2:
+                P, Q = np.ascontiguousarray(P[:, :2]), np.ascontiguousarray(Q[:, :2])
+        else:
+            P = np.cumsum(rng.standard_normal((m, d)), axis=0)
+            Q = np.cumsum(rng.standard_normal((n, d)), axis=0)
+        pairs.append((P, Q))
+    pairs.append((np.array([[0.0, 0.0]]), np.array([[1.0, 0.0], [0.0, 2.0]])))  # 1-point curve
+    pairs.append((np.array([[0.0, 0.0], [1.0, 1.0]]), np.array([[0.5, 0.0], [1.0, 2.0]])))
+    return pairs
+
+
+@pytest.fixture(scope="module")
+def batch_and_ref(gpu):
+    pairs = mixed_pairs(40, seed=5)
+    refs = [O.distance(P, Q, stop_mode=1, eps=1e-9) for P, Q in pairs]
+    b = Batch(pairs, ctx=gpu)
+    yield b, refs
+    b.close()
+
+
+@pytest.mark.parametrize("rows", [1, 2, 4])
+def test_batch_matches_oracle_per_pair(batch_and_ref, rows):
+    b, refs = batch_and_ref
+    print("batch rows", rows, flush=True)
+    out = b.run(stop="rel", eps=1e-9, rows_per_lane=rows)
+    for k, ref in enumerate(refs):
+        assert out["dfd"][k] == ref["dfd"], k
+        assert out["vefd"][k] == ref["vefd"], k
+        assert out["iterations"][k] == ref["iterations"], k
+        assert out["distance"][k] == ref["distance"], k
+    assert sum(r["iterations"] for r in refs) > 0  # some pairs bisect
+
+
+def test_batch_graph_replay_and_stream_agree(batch_and_ref):
+    b, refs = batch_and_ref
+    a = b.run(rows_per_lane=2, use_graph=True)
+    c = b.run(rows_per_lane=2, use_graph=True)
+    d = b.run(rows_per_lane=2, use_graph=False)
+    for key in ("distance", "dfd", "vefd", "iterations", "lb", "ub"):
+        assert np.array_equal(a[key], c[key]), key
+        assert np.array_equal(a[key], d[key]), key
+    assert (a["latency_ms"] > 0).all()
+
+
+def test_batch_pruning_does_not_change_results(batch_and_ref):
+    b, refs = batch_and_ref
+    on = b.run(rows_per_lane=4, prune=1)
+    off = b.run(rows_per_lane=4, prune=0)
+    for key in ("distance", "dfd", "vefd", "iterations"):
+        assert np.array_equal(on[key], off[key]), key
+
+
+def test_batch_reference_stop_rule(batch_and_ref):
+    b, refs = batch_and_ref
+    pairs = mixed_pairs(40, seed=5)
+    eps = float(np.finfo(np.float32).eps)
+    out = b.run(stop="abs", eps=eps, rows_per_lane=2)
+    for k in range(0, len(pairs), 5):
+        ref = O.distance(*pairs[k], stop_mode=0, eps=eps)
+        assert out["distance"][k] == ref["distance"], k
+        assert out["iterations"][k] == ref["iterations"], k
+
+
+def test_batch_autotuner_picks_a_valid_height(batch_and_ref):
+    b, _ = batch_and_ref
+    best, ms = b.tune()
+    assert best in (1, 2, 4)
+    assert all(t > 0 for t in ms)
+    assert ms[[1, 2, 4].index(best)] == min(ms)
