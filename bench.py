@@ -63,6 +63,39 @@ def cpu_baseline(P, Q, mix: dict, dims: int, budget_s: float = 12.0) -> dict:
                       "; rates applied to the GPU run's per-sweep cell counts"}
 
 
+def batch_section(count: int, ctx) -> dict:
+    """Pairs/s of the batched pipeline on `count` pairs (lengths log-uniform in
+    [500, 20000] per curve, dims 2/3 at random, N(0,1) random walks, seed 0),
+    for sequential bisection and for speculative bisection (same results)."""
+    import numpy as np
+    from robotfd_amd import workloads as W
+    from robotfd_amd.batch import Batch
+
+    pairs = W.batch_pairs(count, seed=0)
+    res = {"workload": f"{count} pairs, lengths log-uniform [500, 20000] per curve, "
+                       "dims 2/3 50/50, N(0,1) random walks, seed 0; to 1e-9 relative"}
+    with Batch(pairs, ctx=ctx) as b:
+        best, tune_ms = b.tune()
+        res["rows_per_lane"] = best
+        res["tune_ms"] = tune_ms
+        ref = None
+        for name, depth in (("plain", 1), ("speculative", 0)):
+            b.run(spec_depth=depth)  # warm
+            t0 = time.perf_counter()
+            r = b.run(spec_depth=depth)
+            dt = time.perf_counter() - t0
+            if ref is None:
+                ref = r
+            lat = r["latency_ms"]
+            res[name] = {"pairs_per_s": count / dt, "ms": dt * 1e3,
+                         "p50_latency_ms": float(np.percentile(lat, 50)),
+                         "p99_latency_ms": float(np.percentile(lat, 99)),
+                         "bisected_pairs": int((r["iterations"] > 0).sum()),
+                         "iterations": int(r["iterations"].sum()),
+                         "identical_to_plain": bool(np.array_equal(r["distance"], ref["distance"]))}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +104,8 @@ def main() -> int:
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--points", type=int, default=200_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-pairs", type=int, default=4096,
+                    help="pairs of the batched workload reported beside the main line (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path has no "
@@ -132,10 +167,12 @@ def main() -> int:
         "kernels": {k: dict(v, avg_ms=v["ms"] / v["launches"],
                             cells_per_s=v["cells"] / (v["ms"] * 1e-3)) for k, v in stats.items()},
     }
+    sess.free_memory()
+    if args.batch_pairs:
+        out["batch"] = batch_section(args.batch_pairs, ctx)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(P, Q, stats, dims)
     print(json.dumps(out), flush=True)
-    sess.free_memory()
     return 0
 
 

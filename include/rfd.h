@@ -51,6 +51,8 @@ typedef struct {
     int32_t rows_per_lane; /* sweep tile height per lane: 0 = library choice, else 1, 2 or 4 */
     int32_t use_graph;     /* batch: 1 = replay the captured hipGraph of all rounds */
     int32_t prune;         /* VE-FD cells above DFD^2 skipped: -1 = library default, 0, 1 */
+    int32_t spec_depth;    /* batch: levels of the bisection tree one decision round may
+                              evaluate when few pairs remain (0 = library default, 1 = plain) */
 } rfd_opts;
 
 typedef struct {

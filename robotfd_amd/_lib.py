@@ -29,7 +29,7 @@ class Opts(ctypes.Structure):
                 ("eps", ctypes.c_double), ("vefd_gate", ctypes.c_int32),
                 ("verbose", ctypes.c_int32), ("mixed_switch", ctypes.c_double),
                 ("rows_per_lane", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-                ("prune", ctypes.c_int32)]
+                ("prune", ctypes.c_int32), ("spec_depth", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):

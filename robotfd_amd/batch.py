@@ -54,12 +54,14 @@ class Batch:
         return r.value, list(ms)
 
     def run(self, *, stop: str = "rel", eps: float = 1e-9, vefd_gate: bool = True,
-            rows_per_lane: int = 0, use_graph: bool = True, prune: int = -1) -> dict:
+            rows_per_lane: int = 0, use_graph: bool = True, prune: int = -1,
+            spec_depth: int = 0) -> dict:
         """Returns arrays: distance, dfd, vefd, lb, ub, iterations, latency_ms, ..."""
         o = _opts(stop=stop, eps=eps, vefd_gate=vefd_gate)
         o.rows_per_lane = rows_per_lane
         o.use_graph = 1 if use_graph else 0
         o.prune = prune
+        o.spec_depth = spec_depth
         res = (_lib.Result * self.npairs)()
         lat = (ctypes.c_double * self.npairs)()
         check(lib().rfd_batch_run(self.handle, ctypes.byref(o), res, lat), self.ctx.handle)

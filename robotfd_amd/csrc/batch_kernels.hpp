@@ -19,12 +19,24 @@ namespace rfd {
 
 enum BatchPhase { PH_DFD = 0, PH_VEFD = 1, PH_VEFD_REDO = 2, PH_DEC = 3 };
 
+// Speculative bisection.  A decision round may evaluate the next d levels of a
+// pair's bisection tree at once: node 0 splits (lb, ub); node i's children
+// 2i+1 and 2i+2 split the lower and upper half it leaves.  Every node's radius
+// is computed exactly as the sequential loop would compute it on reaching that
+// node, so walking the tree with the decisions reproduces the sequential
+// trajectory bit for bit, in d times fewer rounds.  d is chosen per round
+// from how many bands the active pairs have: speculation fills waves that
+// would otherwise idle when few pairs remain.
+constexpr int kSpec = 8;      // tree nodes per pair per round (depth <= 3 uses 7)
+constexpr int kMaxDepth = 3;
+
 struct BatchPair {
     long long m, n;          // points
     long long poff, qoff;    // offsets (in doubles) of P and Q in the point pool (3 per point)
     long long qsoff;         // offset of the decision's per-column terms (2 per column)
     long long sloff;         // offset of the two boundary columns (2 * n)
-    long long bandoff;       // offset of this pair's progress flags
+    long long bandoff;       // offset of this pair's progress flags (kSpec * bandstride)
+    long long bandstride;    // progress flags per tree node
     double endpoint_lb;      // frechet.cpp:30, computed on the host
     int big;                 // m*n > 10000 (.cu:1343, in 64 bits)
     int pad;
@@ -32,7 +44,8 @@ struct BatchPair {
 
 struct BatchState {
     double lb, ub, split, dfd, vefd, F, distance;
-    double result[2];        // terminal value of the pair's last sweep
+    double result[kSpec][2]; // terminal value of each tree node's last sweep (node 0: bounds)
+    double radius[kSpec];    // decision radius of each tree node this round
     double thr;              // VE-FD prune threshold used
     long long iterations;
     unsigned long long t_done;  // wall clock when the pair's result became final
@@ -50,7 +63,7 @@ struct BatchArgs {
     double *slots;
     int *progress;
     int2 *items;             // (pair, band) work list of the current round
-    int *count;              // [0] items, [1] queue
+    int *count;              // [0] items, [1] queue, [2] speculation depth of the round
     unsigned long long *evals;
     unsigned long long *t0;  // wall clock at batch start
     int rows_per_lane;
@@ -58,6 +71,8 @@ struct BatchArgs {
     double eps;
     int prune;
     int gate;                // 1: VE-FD only when m*n > 10000
+    int max_waves;           // waves of a sweep launch (sets the speculation depth)
+    int max_depth;           // 1 = plain sequential bisection
 };
 
 __device__ inline bool batch_keep(double lb, double ub, int stop_mode, double eps) {
@@ -94,7 +109,7 @@ __global__ void batch_reset(BatchArgs a) {
     st.vefd = p.endpoint_lb;
     st.F = 0;
     st.distance = 0;
-    st.result[0] = st.result[1] = -3.0;
+    for (int q = 0; q < kSpec; ++q) st.result[q][0] = st.result[q][1] = -3.0;
     st.thr = INFINITY;
     st.iterations = 0;
     st.t_done = 0;
@@ -103,10 +118,36 @@ __global__ void batch_reset(BatchArgs a) {
     if (k == 0) *a.evals = 0;
 }
 
+// Radii of the speculative tree of pair state st, depth d; returns how many
+// nodes need a sweep (nodes whose interval still fails the stop rule are
+// skipped, and so are their descendants).  node_ok receives a mask.
+__device__ inline unsigned spec_tree(BatchState &st, int d, int stop_mode, double eps) {
+    double lo[kSpec], hi[kSpec];
+    unsigned ok = 0;
+    lo[0] = st.lb;
+    hi[0] = st.ub;
+    const int nodes = (1 << d) - 1;
+    for (int i = 0; i < nodes; ++i) {
+        if (i > 0) {
+            const int par = (i - 1) / 2;
+            if (!(ok & (1u << par))) continue;
+            const double sp = st.radius[par];
+            if (i == 2 * par + 1) { lo[i] = lo[par]; hi[i] = sp; }   // par reachable: ub = split
+            else { lo[i] = sp; hi[i] = hi[par]; }                   // not reachable: lb = split
+        }
+        if (!batch_keep(lo[i], hi[i], stop_mode, eps)) continue;
+        st.radius[i] = (hi[i] + lo[i]) / 2.0;  // frechet.cpp:42
+        ok |= 1u << i;
+    }
+    return ok;
+}
+
 // One workgroup of 1024 threads: exclusive scan of per-pair band counts in
 // `order`, then every thread writes its pairs' items and clears their flags.
+// An item is (pair * kSpec + node, band).
 __global__ __launch_bounds__(1024) void batch_plan(BatchArgs a, int phase) {
     __shared__ long long part[1024];
+    __shared__ int depth_s;
     const int tid = (int)threadIdx.x;
     const int per = (a.npairs + 1023) / 1024;
     const int k0 = tid * per;
@@ -130,25 +171,70 @@ __global__ __launch_bounds__(1024) void batch_plan(BatchArgs a, int phase) {
         part[tid] += v;
         __syncthreads();
     }
-    long long pos = part[tid] - mine;
+    if (tid == 0) {
+        int d = 1;
+        if (phase == PH_DEC) {
+            const long long bands = part[1023];
+            for (int c = a.max_depth; c > 1; --c)
+                if (bands * ((1 << c) - 1) <= 2LL * a.max_waves) { d = c; break; }
+        }
+        depth_s = d;
+        a.count[2] = d;
+    }
+    __syncthreads();
+    const int d = depth_s;
+    // Second pass: per-pair node counts (d > 1 changes them), then a second scan.
+    long long mine2 = 0;
     for (int q = 0; q < per; ++q) {
         const int idx = k0 + q;
         if (idx >= a.npairs) break;
         const int k = a.order[idx];
         const BatchPair &p = a.pairs[k];
-        if (!batch_in_round(p, a.state[k], phase, a.gate)) continue;
+        BatchState &st = a.state[k];
+        if (!batch_in_round(p, st, phase, a.gate)) continue;
+        const long long rows = batch_rows(p, phase);
+        const long long nb = (rows + 64LL * R - 1) / (64LL * R);
+        unsigned ok = 1;
+        if (phase == PH_DEC) ok = spec_tree(st, d, a.stop_mode, a.eps);
+        mine2 += nb * __builtin_popcount(ok);
+    }
+    __syncthreads();
+    part[tid] = mine2;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        long long v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    long long pos = part[tid] - mine2;
+    for (int q = 0; q < per; ++q) {
+        const int idx = k0 + q;
+        if (idx >= a.npairs) break;
+        const int k = a.order[idx];
+        const BatchPair &p = a.pairs[k];
+        BatchState &st = a.state[k];
+        if (!batch_in_round(p, st, phase, a.gate)) continue;
         const long long rows = batch_rows(p, phase);
         const int nb = (int)((rows + 64LL * R - 1) / (64LL * R));
-        for (int b = 0; b < nb; ++b) {
-            a.items[pos + b] = make_int2(k, b);
-            a.progress[p.bandoff + b] = 0;
+        unsigned ok = 1;
+        if (phase == PH_DEC) {
+            // spec_tree is deterministic in (lb, ub): recompute the mask.
+            ok = spec_tree(st, d, a.stop_mode, a.eps);
         }
-        pos += nb;
-        a.state[k].result[0] = -3.0;
-        a.state[k].result[1] = -3.0;
+        for (int node = 0; node < kSpec; ++node) {
+            if (!(ok & (1u << node))) continue;
+            for (int b = 0; b < nb; ++b) {
+                a.items[pos + b] = make_int2(k * kSpec + node, b);
+                a.progress[p.bandoff + node * p.bandstride + b] = 0;
+            }
+            pos += nb;
+            st.result[node][0] = -3.0;
+            st.result[node][1] = -3.0;
+        }
         if (phase == PH_VEFD || phase == PH_VEFD_REDO) {
-            const double F = a.state[k].F;
-            a.state[k].thr = (phase == PH_VEFD && a.prune) ? F * (1.0 + 0x1p-40) : INFINITY;
+            const double F = st.F;
+            st.thr = (phase == PH_VEFD && a.prune) ? F * (1.0 + 0x1p-40) : INFINITY;
         }
     }
     if (tid == 1023) {
@@ -170,7 +256,8 @@ __global__ __launch_bounds__(64) void batch_sweep(BatchArgs a) {
         idx = __builtin_amdgcn_readfirstlane(idx);
         if (idx >= total) break;
         const int2 it = a.items[idx];
-        const int k = __builtin_amdgcn_readfirstlane(it.x);
+        const int kn = __builtin_amdgcn_readfirstlane(it.x);
+        const int k = kn / kSpec, node = kn % kSpec;
         const int band = __builtin_amdgcn_readfirstlane(it.y);
         const BatchPair p = a.pairs[k];
         BatchState &st = a.state[k];
@@ -180,13 +267,13 @@ __global__ __launch_bounds__(64) void batch_sweep(BatchArgs a) {
         job.P = a.pts + p.poff;
         job.Q = a.pts + p.qoff;
         job.qseg = a.qseg + p.qsoff;
-        job.slots = a.slots + p.sloff;
-        job.progress = a.progress + p.bandoff;
-        job.result = st.result;
+        job.slots = a.slots + p.sloff + (long long)node * 2 * p.n;
+        job.progress = a.progress + p.bandoff + (long long)node * p.bandstride;
+        job.result = st.result[node];
         job.rows = rows;
         job.cols = cols;
         job.nbands = (int)((rows + 64LL * R - 1) / (64LL * R));
-        job.param = (OP == OP_DEC) ? st.split : (OP == OP_VEFD ? p.endpoint_lb : 0.0);
+        job.param = (OP == OP_DEC) ? st.radius[node] : (OP == OP_VEFD ? p.endpoint_lb : 0.0);
         job.thr = (OP == OP_VEFD) ? st.thr : INFINITY;
         run_band<OP, 3, double, SYNC_SCOPED, R>(job, band, qring, sring, evals, nullptr);
     }
@@ -215,8 +302,8 @@ __global__ void batch_update(BatchArgs a, int phase) {
     BatchState &st = a.state[k];
     if (!batch_in_round(p, st, phase, a.gate)) return;
     if (phase == PH_DFD) {
-        st.dfd = st.result[0];
-        st.F = st.result[1];
+        st.dfd = st.result[0][0];
+        st.F = st.result[0][1];
         st.ub = st.dfd;
         st.lb = p.endpoint_lb;
         if (p.m < 2 || p.n < 2) {
@@ -236,17 +323,25 @@ __global__ void batch_update(BatchArgs a, int phase) {
         return;
     }
     if (phase == PH_VEFD || phase == PH_VEFD_REDO) {
-        if (phase == PH_VEFD && st.result[1] >= st.thr) {  // pruning reached: rerun unpruned
+        if (phase == PH_VEFD && st.result[0][1] >= st.thr) {  // pruning reached: rerun unpruned
             st.need_redo = 1;
             return;
         }
         st.need_redo = 0;
-        st.vefd = st.result[0];
+        st.vefd = st.result[0][0];
         st.lb = st.vefd;
     } else {
-        const bool reach = st.result[0] == 1.0;  // .cu:1592
-        if (reach) st.ub = st.split; else st.lb = st.split;
-        st.iterations += 1;
+        // Walk the speculative tree with its decisions (sequential bisection order).
+        const int d = a.count[2];
+        int node = 0;
+        for (int level = 0; level < d; ++level) {
+            if (!batch_keep(st.lb, st.ub, a.stop_mode, a.eps)) break;
+            const double split = (st.ub + st.lb) / 2.0;  // frechet.cpp:42 (== radius[node])
+            const bool reach = st.result[node][0] == 1.0;  // .cu:1592
+            if (reach) st.ub = split; else st.lb = split;
+            st.iterations += 1;
+            node = reach ? 2 * node + 1 : 2 * node + 2;
+        }
     }
     st.active = batch_keep(st.lb, st.ub, a.stop_mode, a.eps) ? 1 : 0;
     st.split = (st.ub + st.lb) / 2.0;     // frechet.cpp:42

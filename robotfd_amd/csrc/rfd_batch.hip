@@ -32,11 +32,11 @@ constexpr int kDecisionRounds = 48;
 constexpr int kContinueRounds = 16;
 
 struct GraphKey {
-    int rpl, stop_mode, prune, gate, rounds, with_bounds;
+    int rpl, stop_mode, prune, gate, rounds, with_bounds, depth;
     double eps;
     bool operator<(const GraphKey &o) const {
-        return std::tie(rpl, stop_mode, prune, gate, rounds, with_bounds, eps) <
-               std::tie(o.rpl, o.stop_mode, o.prune, o.gate, o.rounds, o.with_bounds, o.eps);
+        return std::tie(rpl, stop_mode, prune, gate, rounds, with_bounds, depth, eps) <
+               std::tie(o.rpl, o.stop_mode, o.prune, o.gate, o.rounds, o.with_bounds, o.depth, o.eps);
     }
 };
 
@@ -97,6 +97,8 @@ rfd::BatchArgs make_args(rfd_batch *b, int rpl, const rfd_opts &o, int prune) {
     a.eps = o.eps;
     a.prune = prune;
     a.gate = o.vefd_gate;
+    a.max_waves = b->ctx->max_waves;
+    a.max_depth = o.spec_depth >= 1 ? std::min(o.spec_depth, rfd::kMaxDepth) : rfd::kMaxDepth;
     return a;
 }
 
@@ -152,7 +154,7 @@ rfd_status run_once(rfd_batch *b, const rfd_opts &o, int rpl, int prune,
     RFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (o.use_graph) {
         hipGraphExec_t ge;
-        GraphKey key{rpl, o.stop_mode, prune, o.vefd_gate, kDecisionRounds, 1, o.eps};
+        GraphKey key{rpl, o.stop_mode, prune, o.vefd_gate, kDecisionRounds, 1, a.max_depth, o.eps};
         rfd_status s = get_graph(b, key, a, &ge);
         if (s != RFD_OK) return s;
         RFD_HIP(ctx, hipGraphLaunch(ge, ctx->stream));
@@ -174,7 +176,7 @@ rfd_status run_once(rfd_batch *b, const rfd_opts &o, int rpl, int prune,
         }
         if (o.use_graph) {
             hipGraphExec_t ge;
-            GraphKey key{rpl, o.stop_mode, prune, o.vefd_gate, kContinueRounds, 0, o.eps};
+            GraphKey key{rpl, o.stop_mode, prune, o.vefd_gate, kContinueRounds, 0, a.max_depth, o.eps};
             rfd_status s = get_graph(b, key, a, &ge);
             if (s != RFD_OK) return s;
             RFD_HIP(ctx, hipGraphLaunch(ge, ctx->stream));
@@ -250,9 +252,10 @@ rfd_status rfd_batch_create(rfd_ctx *ctx, int64_t npairs, const double *const *P
         p.qsoff = qsofs;
         qsofs += 2 * std::max<int64_t>(n[k] - 1, 1);
         p.sloff = slofs;
-        slofs += 2 * n[k];
+        slofs += (long long)rfd::kSpec * 2 * n[k];
         p.bandoff = bofs;
-        bofs += (m[k] + 63) / 64 + 1;
+        p.bandstride = (m[k] + 63) / 64 + 1;
+        bofs += (long long)rfd::kSpec * p.bandstride;
         p.endpoint_lb = rfd_host::endpoint_bound(P[k], m[k], Q[k], n[k], dims[k]);
         p.big = (m[k] * n[k] > 10000) ? 1 : 0;
         p.pad = 0;

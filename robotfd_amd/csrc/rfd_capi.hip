@@ -234,6 +234,7 @@ void rfd_default_opts(rfd_opts *o) {
     o->rows_per_lane = 0;
     o->use_graph = 1;
     o->prune = -1;
+    o->spec_depth = 0;
 }
 
 rfd_status rfd_create(rfd_ctx **out, int device) {

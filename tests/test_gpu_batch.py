@@ -41,11 +41,13 @@ def batch_and_ref(gpu):
     b.close()
 
 
-@pytest.mark.parametrize("rows", [1, 2, 4])
-def test_batch_matches_oracle_per_pair(batch_and_ref, rows):
+@pytest.mark.parametrize("rows,depth", [(1, 1), (2, 0), (4, 0), (4, 2), (2, 3)])
+def test_batch_matches_oracle_per_pair(batch_and_ref, rows, depth):
+    """Also pins speculative bisection: walking the evaluated tree must give
+    the sequential trajectory (same iterations, same final bracket)."""
     b, refs = batch_and_ref
-    print("batch rows", rows, flush=True)
-    out = b.run(stop="rel", eps=1e-9, rows_per_lane=rows)
+    print("batch rows", rows, "depth", depth, flush=True)
+    out = b.run(stop="rel", eps=1e-9, rows_per_lane=rows, spec_depth=depth)
     for k, ref in enumerate(refs):
         assert out["dfd"][k] == ref["dfd"], k
         assert out["vefd"][k] == ref["vefd"], k
