@@ -148,6 +148,43 @@ rfd_status rfd_batch_run(rfd_batch *b, const rfd_opts *opts, rfd_result *results
 rfd_status rfd_batch_tune(rfd_batch *b, const rfd_opts *opts, int32_t *rows_per_lane, double *ms);
 rfd_status rfd_batch_destroy(rfd_batch *b);
 
+/* ---- Bracket-position network (learned bracket) ----
+ * No counterpart in the reference.  A BF16 MLP on MFMA (386 features -> 4 x 256
+ * ReLU -> sigmoid) predicts where the distance lies in [VE-FD, DFD] as a
+ * fraction f.  A learned batch run certifies a bracket around the prediction
+ * for each bisecting pair and falls back to bisection of whatever bracket the
+ * certifying decisions leave when the prediction misses.  Policy: f < tau ->
+ * [lb, lb + t] (one decision; t = half the stop tolerance, so a success ends
+ * the pair), f > 1 - tau -> [ub - t, ub], otherwise [f -/+ delta] of the
+ * bracket (two decisions) when delta > 0. */
+typedef struct rfd_mlp rfd_mlp;
+
+int64_t rfd_mlp_param_count(void);
+rfd_status rfd_mlp_create(rfd_ctx *ctx, uint64_t seed, rfd_mlp **out);
+rfd_status rfd_mlp_destroy(rfd_mlp *net);
+/* FP32 parameters: W1[400x256] b1 W2[256x256] b2 W3 b3 W4 b4 w5[256] b5, row-major. */
+rfd_status rfd_mlp_get_params(rfd_mlp *net, float *out);
+rfd_status rfd_mlp_set_params(rfd_mlp *net, const float *in);
+rfd_status rfd_mlp_save(rfd_mlp *net, const char *path);
+rfd_status rfd_mlp_load(rfd_mlp *net, const char *path);
+rfd_status rfd_mlp_set_policy(rfd_mlp *net, double tau, double delta);
+rfd_status rfd_mlp_get_policy(rfd_mlp *net, double *tau, double *delta);
+/* features: rows x 386 FP32; out: rows sigmoid outputs. */
+rfd_status rfd_mlp_forward(rfd_mlp *net, const float *features, int64_t rows, float *out);
+/* Binary cross-entropy against labels in [0,1]; grads: rfd_mlp_param_count() FP32. */
+rfd_status rfd_mlp_gradients(rfd_mlp *net, const float *features, const float *labels,
+                             int64_t rows, float *grads, float *loss);
+/* Adam training on the device; loss_per_epoch (may be NULL) gets `epochs` values. */
+rfd_status rfd_mlp_train(rfd_mlp *net, const float *features, const float *labels, int64_t rows,
+                         int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                         float *loss_per_epoch);
+/* Features of every pair from the bounds of the batch's last run (npairs x 386). */
+rfd_status rfd_batch_features(rfd_batch *b, float *out);
+/* rfd_batch_run with the network's predicted bracket certified after the
+ * bounds; certified (may be NULL) gets 1 for pairs whose prediction held. */
+rfd_status rfd_batch_run_learned(rfd_batch *b, rfd_mlp *net, const rfd_opts *opts,
+                                 rfd_result *results, double *latency_ms, int32_t *certified);
+
 #ifdef __cplusplus
 }
 #endif

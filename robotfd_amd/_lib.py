@@ -60,6 +60,7 @@ _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _i64 = ctypes.c_int64
+_fp = ctypes.POINTER(ctypes.c_float)
 SIGNATURES = {
     "rfd_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int]),
     "rfd_destroy": (ctypes.c_int, [_vp]),
@@ -82,6 +83,22 @@ SIGNATURES = {
     "rfd_batch_run": (ctypes.c_int, [_vp, ctypes.POINTER(Opts), ctypes.POINTER(Result), _dp]),
     "rfd_batch_tune": (ctypes.c_int, [_vp, ctypes.POINTER(Opts), ctypes.POINTER(ctypes.c_int32), _dp]),
     "rfd_batch_destroy": (ctypes.c_int, [_vp]),
+    "rfd_mlp_param_count": (ctypes.c_int64, []),
+    "rfd_mlp_create": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    "rfd_mlp_destroy": (ctypes.c_int, [_vp]),
+    "rfd_mlp_get_params": (ctypes.c_int, [_vp, _fp]),
+    "rfd_mlp_set_params": (ctypes.c_int, [_vp, _fp]),
+    "rfd_mlp_save": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "rfd_mlp_load": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "rfd_mlp_set_policy": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double]),
+    "rfd_mlp_get_policy": (ctypes.c_int, [_vp, _dp, _dp]),
+    "rfd_mlp_forward": (ctypes.c_int, [_vp, _fp, _i64, _fp]),
+    "rfd_mlp_gradients": (ctypes.c_int, [_vp, _fp, _fp, _i64, _fp, _fp]),
+    "rfd_mlp_train": (ctypes.c_int, [_vp, _fp, _fp, _i64, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.c_float, ctypes.c_uint64, _fp]),
+    "rfd_batch_features": (ctypes.c_int, [_vp, _fp]),
+    "rfd_batch_run_learned": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(Opts), ctypes.POINTER(Result),
+                                             _dp, ctypes.POINTER(ctypes.c_int32)]),
 }
 
 _LIB = None
@@ -123,3 +140,8 @@ def as_curve(a) -> np.ndarray:
 
 def dptr(a: np.ndarray):
     return a.ctypes.data_as(_dp)
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_fp)

@@ -19,7 +19,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "librfd.so")
 
-HIP_SOURCES = ["rfd_capi.hip", "rfd_batch.hip"]
+HIP_SOURCES = ["rfd_capi.hip", "rfd_batch.hip", "rfd_mlp.hip"]
 CXX_SOURCES = ["wrapper.cpp"]
 
 HIPCC_FLAGS = [

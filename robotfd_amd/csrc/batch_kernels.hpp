@@ -16,8 +16,18 @@
 #include "sweep_kernels.hpp"
 
 namespace rfd {
+constexpr int MLP_FEAT_ROW = 386;  // features per pair (see mlp_kernels.hpp)
+constexpr int MLP_IN_ROW = 400;    // padded network input
+__device__ inline uint16_t f2bf_dev(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+}  // namespace rfd
 
-enum BatchPhase { PH_DFD = 0, PH_VEFD = 1, PH_VEFD_REDO = 2, PH_DEC = 3 };
+namespace rfd {
+
+enum BatchPhase { PH_DFD = 0, PH_VEFD = 1, PH_VEFD_REDO = 2, PH_DEC = 3, PH_CERT = 4 };
 
 // Speculative bisection.  A decision round may evaluate the next d levels of a
 // pair's bisection tree at once: node 0 splits (lb, ub); node i's children
@@ -51,6 +61,8 @@ struct BatchState {
     unsigned long long t_done;  // wall clock when the pair's result became final
     int active;
     int need_redo;
+    unsigned cert_mask;      // learned run: nodes 0 (lower radius) and 1 (upper radius) to certify
+    int certified;           // learned run: the predicted bracket held
 };
 
 struct BatchArgs {
@@ -93,6 +105,7 @@ __device__ inline bool batch_in_round(const BatchPair &p, const BatchState &st, 
     if (p.m < 2 || p.n < 2) return false;
     if (phase == PH_VEFD) return batch_runs_vefd(p, gate);
     if (phase == PH_VEFD_REDO) return st.need_redo != 0;
+    if (phase == PH_CERT) return st.active != 0 && st.cert_mask != 0;
     return st.active != 0;
 }
 
@@ -115,6 +128,8 @@ __global__ void batch_reset(BatchArgs a) {
     st.t_done = 0;
     st.active = 0;
     st.need_redo = 0;
+    st.cert_mask = 0;
+    st.certified = 0;
     if (k == 0) *a.evals = 0;
 }
 
@@ -196,6 +211,7 @@ __global__ __launch_bounds__(1024) void batch_plan(BatchArgs a, int phase) {
         const long long nb = (rows + 64LL * R - 1) / (64LL * R);
         unsigned ok = 1;
         if (phase == PH_DEC) ok = spec_tree(st, d, a.stop_mode, a.eps);
+        if (phase == PH_CERT) ok = st.cert_mask;
         mine2 += nb * __builtin_popcount(ok);
     }
     __syncthreads();
@@ -222,6 +238,7 @@ __global__ __launch_bounds__(1024) void batch_plan(BatchArgs a, int phase) {
             // spec_tree is deterministic in (lb, ub): recompute the mask.
             ok = spec_tree(st, d, a.stop_mode, a.eps);
         }
+        if (phase == PH_CERT) ok = st.cert_mask;
         for (int node = 0; node < kSpec; ++node) {
             if (!(ok & (1u << node))) continue;
             for (int b = 0; b < nb; ++b) {
@@ -330,6 +347,24 @@ __global__ void batch_update(BatchArgs a, int phase) {
         st.need_redo = 0;
         st.vefd = st.result[0][0];
         st.lb = st.vefd;
+    } else if (phase == PH_CERT) {
+        // Decisions at the predicted bracket [lo, hi]: certified when lo is
+        // unreachable and hi reachable; otherwise the bracket still shrinks to
+        // the side the decisions point to and bisection continues from there.
+        const bool has_lo = st.cert_mask & 1u, has_hi = st.cert_mask & 2u;
+        const bool rlo = has_lo && st.result[0][0] == 1.0;
+        const bool rhi = !has_hi || st.result[1][0] == 1.0;
+        st.iterations += (has_lo ? 1 : 0) + (has_hi ? 1 : 0);
+        if (rlo) {
+            st.ub = st.radius[0];
+        } else if (!rhi) {
+            st.lb = st.radius[1];
+        } else {
+            if (has_lo) st.lb = st.radius[0];
+            if (has_hi) st.ub = st.radius[1];
+            st.certified = 1;
+        }
+        st.cert_mask = 0;
     } else {
         // Walk the speculative tree with its decisions (sequential bisection order).
         const int d = a.count[2];
@@ -347,6 +382,91 @@ __global__ void batch_update(BatchArgs a, int phase) {
     st.split = (st.ub + st.lb) / 2.0;     // frechet.cpp:42
     st.distance = (st.ub + st.lb) / 2.0;  // frechet.cpp:63
     if (!st.active) st.t_done = wall_clock64();
+}
+
+// Features of pair k, one workgroup of 128 threads: both curves resampled to
+// 64 points by index (linear interpolation), centred on the mean of the 128
+// samples, scaled by 1/DFD; then VE-FD/DFD and endpoint/DFD.  FP32 row of 386
+// (for training records) and BF16 row of 400 (the network's input).
+__global__ __launch_bounds__(128) void batch_features(BatchArgs a, float *fout, uint16_t *xout) {
+    __shared__ double red[3][128];
+    const int k = (int)blockIdx.x;
+    const int t = (int)threadIdx.x;
+    if (k >= a.npairs) return;
+    const BatchPair p = a.pairs[k];
+    const BatchState &st = a.state[k];
+    const double *C = a.pts + (t < 64 ? p.poff : p.qoff);
+    const long long len = t < 64 ? p.m : p.n;
+    const int s = t & 63;
+    double x[3];
+    if (len > 1) {
+        const double pos = (double)s * (double)(len - 1) / 63.0;
+        long long i0 = (long long)pos;
+        if (i0 > len - 2) i0 = len - 2;
+        const double f = pos - (double)i0;
+        for (int c = 0; c < 3; ++c) x[c] = C[3 * i0 + c] + f * (C[3 * (i0 + 1) + c] - C[3 * i0 + c]);
+    } else {
+        for (int c = 0; c < 3; ++c) x[c] = C[c];
+    }
+    for (int c = 0; c < 3; ++c) red[c][t] = x[c];
+    __syncthreads();
+    for (int off = 64; off > 0; off >>= 1) {
+        if (t < off)
+            for (int c = 0; c < 3; ++c) red[c][t] += red[c][t + off];
+        __syncthreads();
+    }
+    const double scale = st.dfd > 0 ? st.dfd : 1.0;
+    float *fr = fout + (long long)k * MLP_FEAT_ROW;
+    uint16_t *xr = xout + (long long)k * MLP_IN_ROW;
+    for (int c = 0; c < 3; ++c) {
+        const double mean = red[c][0] / 128.0;
+        const float v = (float)((x[c] - mean) / scale);
+        const int col = (t < 64 ? 0 : 192) + s * 3 + c;
+        fr[col] = v;
+        xr[col] = f2bf_dev(v);
+    }
+    if (t == 0) {
+        const float lbf = (float)(st.vefd / scale), epf = (float)(p.endpoint_lb / scale);
+        fr[384] = lbf;
+        fr[385] = epf;
+        xr[384] = f2bf_dev(lbf);
+        xr[385] = f2bf_dev(epf);
+        for (int c = MLP_FEAT_ROW; c < MLP_IN_ROW; ++c) xr[c] = 0;
+    }
+}
+
+// Learned run: the bracket to certify for every pair still bisecting, from
+// the predicted fraction f of [lb, ub] where the distance lies.
+//  f < tau:      [lb, lb + t]  one decision at the top end; t meets the stop rule
+//  f > 1 - tau:  [ub - t, ub]  one decision at the bottom end
+//  otherwise:    [lb + (f - delta) w, lb + (f + delta) w] when delta > 0
+// where t is half the stop tolerance at that end.  Node 0 holds the lower
+// radius, node 1 the upper; a bracket end equal to lb or ub needs no decision.
+__global__ void batch_learned_prepare(BatchArgs a, const float *pred, double tau, double delta) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= a.npairs) return;
+    BatchState &st = a.state[k];
+    st.cert_mask = 0;
+    if (!st.active) return;
+    const double f = (double)pred[k];
+    const double w = st.ub - st.lb;
+    unsigned mask = 0;
+    if (f < tau) {
+        const double t = a.stop_mode == 0 ? 0.5 * a.eps : 0.5 * a.eps * st.lb;
+        st.radius[1] = st.lb + t;
+        if (st.radius[1] > st.lb && st.radius[1] < st.ub) mask = 2u;
+    } else if (f > 1.0 - tau) {
+        const double t = a.stop_mode == 0 ? 0.5 * a.eps : 0.5 * a.eps * st.ub;
+        st.radius[0] = st.ub - t;
+        if (st.radius[0] > st.lb && st.radius[0] < st.ub) mask = 1u;
+    } else if (delta > 0.0) {
+        st.radius[0] = st.lb + (f - delta) * w;
+        st.radius[1] = st.lb + (f + delta) * w;
+        if (st.radius[0] > st.lb && st.radius[0] < st.ub) mask |= 1u;
+        if (st.radius[1] > st.lb && st.radius[1] < st.ub) mask |= 2u;
+        if ((mask & 3u) == 3u && !(st.radius[0] < st.radius[1])) mask = 2u;
+    }
+    st.cert_mask = mask;
 }
 
 }  // namespace rfd

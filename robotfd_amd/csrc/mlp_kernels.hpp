@@ -1,0 +1,185 @@
+// Bracket-position network: BF16 MLP on MFMA with FP32 accumulation.
+//
+//   x (400 = 386 features + zero pad) -> 256 -> 256 -> 256 -> 256 -> 1
+//   hidden layers: a = relu(x W + b) stored in BF16; head: p = sigmoid(a w + b) in FP32.
+//
+// The hidden layers' forward and backward products run through one GEMM
+// kernel on v_mfma_f32_16x16x16_bf16 (64x64 output tile per workgroup of four
+// waves, each wave a 32x32 block of 2x2 MFMA tiles).  The three operand
+// layouts training needs (X W, X^T dZ, dZ W^T) are strides of the same kernel.
+// Epilogues fuse bias + ReLU (forward) and the ReLU mask of the layer below
+// (backward).  Master weights and Adam moments are FP32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "mlp_layout.hpp"
+
+namespace rfd {
+
+typedef short mfma_s4 __attribute__((ext_vector_type(4)));
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+enum GemmEpi { EPI_F32 = 0, EPI_BIAS_RELU_BF16 = 1, EPI_MASK_BF16 = 2 };
+
+struct GemmArgs {
+    const uint16_t *A;   // BF16, element (i, k) at A[i*sai + k*sak]
+    const uint16_t *B;   // BF16, element (k, j) at B[k*sbk + j*sbj]
+    int M, N, K;
+    long long sai, sak, sbk, sbj;
+    float *Cf;           // EPI_F32: C[i*N + j]
+    uint16_t *Cb;        // BF16 epilogues: C[i*N + j]
+    const float *bias;   // EPI_BIAS_RELU_BF16: bias[j]
+    const uint16_t *mask;  // EPI_MASK_BF16: keep where mask[i*N + j] > 0
+};
+
+// MFMA 16x16x16 BF16 operand layout: lane l supplies A[l%16][4*(l/16) + t]
+// and B[4*(l/16) + t][l%16], t = 0..3; it receives D[4*(l/16) + t][l%16].
+template <int EPI>
+__global__ __launch_bounds__(256) void mlp_gemm(GemmArgs g) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int wave = (int)(threadIdx.x >> 6);
+    const int row0 = (int)blockIdx.y * 64 + (wave >> 1) * 32;
+    const int col0 = (int)blockIdx.x * 64 + (wave & 1) * 32;
+    const int r16 = lane & 15, kq = (lane >> 4) * 4;
+    mfma_f4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < g.K; k0 += 16) {
+        mfma_s4 fa[2], fb[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const int i = row0 + a * 16 + r16;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int k = k0 + kq + t;
+                fa[a][t] = (i < g.M && k < g.K) ? (short)g.A[(long long)i * g.sai + (long long)k * g.sak] : (short)0;
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int j = col0 + b * 16 + r16;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int k = k0 + kq + t;
+                fb[b][t] = (j < g.N && k < g.K) ? (short)g.B[(long long)k * g.sbk + (long long)j * g.sbj] : (short)0;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int i = row0 + a * 16 + kq + t;
+                const int j = col0 + b * 16 + r16;
+                if (i >= g.M || j >= g.N) continue;
+                const float v = acc[a][b][t];
+                const long long o = (long long)i * g.N + j;
+                if (EPI == EPI_F32) {
+                    g.Cf[o] = v;
+                } else if (EPI == EPI_BIAS_RELU_BF16) {
+                    const float z = v + g.bias[j];
+                    g.Cb[o] = f2bf(z > 0.f ? z : 0.f);
+                } else {
+                    g.Cb[o] = g.mask[o] > 0 ? f2bf(v) : (uint16_t)0;  // BF16 > 0 as raw bits: sign clear, nonzero
+                }
+            }
+}
+
+// Head: z_i = a4_i . w5 + b5, p_i = sigmoid(z_i).  Training also forms the
+// binary cross-entropy gradient dz_i = (p_i - y_i) / count over valid rows.
+__global__ void mlp_head(const uint16_t *A4, const float *params, int rows, const float *labels,
+                         int valid, float *p_out, float *dz, float *loss_rows) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= rows) return;
+    const float *w5 = params + MLP_W5;
+    float z = params[MLP_B5];
+    for (int k = 0; k < MLP_H; ++k) z += bf2f(A4[(long long)i * MLP_H + k]) * w5[k];
+    const float p = 1.f / (1.f + __expf(-z));
+    if (p_out) p_out[i] = p;
+    if (dz) {
+        const bool ok = i < valid;
+        const float y = ok ? labels[i] : 0.f;
+        dz[i] = ok ? (p - y) / (float)valid : 0.f;
+        if (loss_rows) {
+            // log(1 + e^-|z|) form of the cross-entropy, stable for large |z|.
+            const float l = fmaxf(z, 0.f) - z * y + __logf(1.f + __expf(-fabsf(z)));
+            loss_rows[i] = ok ? l / (float)valid : 0.f;
+        }
+    }
+}
+
+// dA4 = dz w5^T masked by a4 > 0 (BF16, the next GEMMs' operand); dw5 and db5.
+__global__ void mlp_head_back(const uint16_t *A4, const float *params, const float *dz, int rows,
+                              uint16_t *dZ4, float *grads) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (long long)rows * MLP_H) {
+        const int i = (int)(t / MLP_H), k = (int)(t % MLP_H);
+        const float g = dz[i] * params[MLP_W5 + k];
+        dZ4[t] = A4[t] > 0 ? f2bf(g) : (uint16_t)0;
+    }
+    if (t < MLP_H + 1) {
+        float s = 0.f;
+        if (t < MLP_H) {
+            for (int i = 0; i < rows; ++i) s += dz[i] * bf2f(A4[(long long)i * MLP_H + t]);
+            grads[MLP_W5 + t] = s;
+        } else {
+            for (int i = 0; i < rows; ++i) s += dz[i];
+            grads[MLP_B5] = s;
+        }
+    }
+}
+
+// Column sums of a BF16 [rows x cols] matrix (bias gradients).
+__global__ void mlp_colsum(const uint16_t *X, int rows, int cols, float *out) {
+    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= cols) return;
+    float s = 0.f;
+    for (int i = 0; i < rows; ++i) s += bf2f(X[(long long)i * cols + j]);
+    out[j] = s;
+}
+
+__global__ void mlp_adam(float *params, const float *grads, float *m1, float *m2, long long n,
+                         float lr, float b1, float b2, float eps, float c1, float c2) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float g = grads[t];
+    const float a = b1 * m1[t] + (1.f - b1) * g;
+    const float v = b2 * m2[t] + (1.f - b2) * g * g;
+    m1[t] = a;
+    m2[t] = v;
+    params[t] -= lr * (a / c1) / (sqrtf(v / c2) + eps);
+}
+
+// BF16 copies of the hidden layers' weights (the MFMA operands).
+__global__ void mlp_cast_weights(const float *params, uint16_t *wb) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n0 = (long long)MLP_IN * MLP_H, nh = (long long)MLP_H * MLP_H;
+    if (t >= n0 + 3 * nh) return;
+    long long src;
+    if (t < n0) src = t;
+    else {
+        const long long l = 1 + (t - n0) / nh, r = (t - n0) % nh;
+        src = mlp_w_off((int)l) + r;
+    }
+    wb[t] = f2bf(params[src]);
+}
+
+__global__ void mlp_rows_to_bf16(const float *x, int rows, uint16_t *xb) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)rows * MLP_IN) return;
+    const int i = (int)(t / MLP_IN), c = (int)(t % MLP_IN);
+    xb[t] = c < MLP_FEAT ? f2bf(x[(long long)i * MLP_FEAT + c]) : (uint16_t)0;
+}
+
+}  // namespace rfd

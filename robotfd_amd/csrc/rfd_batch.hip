@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "batch_kernels.hpp"
+#include "mlp_internal.hpp"
 #include "rfd.h"
 #include "rfd_internal.hpp"
 
@@ -34,9 +35,12 @@ constexpr int kContinueRounds = 16;
 struct GraphKey {
     int rpl, stop_mode, prune, gate, rounds, with_bounds, depth;
     double eps;
+    const void *net;   // learned runs: the network whose forward pass is captured
+    double tau, delta;
     bool operator<(const GraphKey &o) const {
-        return std::tie(rpl, stop_mode, prune, gate, rounds, with_bounds, depth, eps) <
-               std::tie(o.rpl, o.stop_mode, o.prune, o.gate, o.rounds, o.with_bounds, o.depth, o.eps);
+        return std::tie(rpl, stop_mode, prune, gate, rounds, with_bounds, depth, eps, net, tau, delta) <
+               std::tie(o.rpl, o.stop_mode, o.prune, o.gate, o.rounds, o.with_bounds, o.depth, o.eps,
+                        o.net, o.tau, o.delta);
     }
 };
 
@@ -59,6 +63,9 @@ struct rfd_batch {
     int2 *items = nullptr;
     int *count = nullptr;
     unsigned long long *evals = nullptr, *t0 = nullptr;
+    float *feat = nullptr;     // [npairs x 386] FP32 features of the last bounds
+    uint16_t *xin = nullptr;   // [npairs x 400] BF16 network input
+    float *pred = nullptr;     // [npairs] network output
     int64_t total_items = 0;
     int tuned_rpl = 0;
     std::string signature;
@@ -102,8 +109,14 @@ rfd::BatchArgs make_args(rfd_batch *b, int rpl, const rfd_opts &o, int prune) {
     return a;
 }
 
-// Enqueues the rounds on the stream (captured or not).
-void enqueue_rounds(rfd_batch *b, const rfd::BatchArgs &a, bool with_bounds, int rounds) {
+void enqueue_rounds(rfd_batch *b, const rfd::BatchArgs &a, bool with_bounds, int rounds,
+                    rfd_mlp *net = nullptr, double tau = 0.0, double delta = 0.0);
+
+// Enqueues the rounds on the stream (captured or not).  With a network: after
+// the bounds, features -> forward pass -> predicted bracket -> one round of
+// certifying decisions, then the bisection rounds.
+void enqueue_rounds(rfd_batch *b, const rfd::BatchArgs &a, bool with_bounds, int rounds,
+                    rfd_mlp *net, double tau, double delta) {
     rfd_ctx *ctx = b->ctx;
     hipStream_t st = ctx->stream;
     const unsigned pg = (unsigned)((b->npairs + 255) / 256);
@@ -119,6 +132,16 @@ void enqueue_rounds(rfd_batch *b, const rfd::BatchArgs &a, bool with_bounds, int
             else launch_batch_sweep<rfd::OP_VEFD>(rpl, grid, st, a);
             hipLaunchKernelGGL(rfd::batch_update, dim3(pg), dim3(256), 0, st, a, ph);
         }
+        if (net) {
+            hipLaunchKernelGGL(rfd::batch_features, dim3((unsigned)b->npairs), dim3(128), 0, st, a,
+                               b->feat, b->xin);
+            rfd_host::mlp_forward_dev(net, b->xin, (int)b->npairs, b->pred);
+            hipLaunchKernelGGL(rfd::batch_learned_prepare, dim3(pg), dim3(256), 0, st, a,
+                               (const float *)b->pred, tau, delta);
+            hipLaunchKernelGGL(rfd::batch_plan, dim3(1), dim3(1024), 0, st, a, (int)rfd::PH_CERT);
+            launch_batch_sweep<rfd::OP_DEC>(rpl, grid, st, a);
+            hipLaunchKernelGGL(rfd::batch_update, dim3(pg), dim3(256), 0, st, a, (int)rfd::PH_CERT);
+        }
     }
     for (int r = 0; r < rounds; ++r) {
         hipLaunchKernelGGL(rfd::batch_plan, dim3(1), dim3(1024), 0, st, a, (int)rfd::PH_DEC);
@@ -127,7 +150,8 @@ void enqueue_rounds(rfd_batch *b, const rfd::BatchArgs &a, bool with_bounds, int
     }
 }
 
-rfd_status get_graph(rfd_batch *b, const GraphKey &key, const rfd::BatchArgs &a, hipGraphExec_t *out) {
+rfd_status get_graph(rfd_batch *b, const GraphKey &key, const rfd::BatchArgs &a, hipGraphExec_t *out,
+                     rfd_mlp *net = nullptr) {
     rfd_ctx *ctx = b->ctx;
     auto it = b->graphs.find(key);
     if (it != b->graphs.end()) {
@@ -136,7 +160,7 @@ rfd_status get_graph(rfd_batch *b, const GraphKey &key, const rfd::BatchArgs &a,
     }
     hipGraph_t g = nullptr;
     RFD_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_rounds(b, a, key.with_bounds != 0, key.rounds);
+    enqueue_rounds(b, a, key.with_bounds != 0, key.rounds, net, key.tau, key.delta);
     RFD_HIP(ctx, hipStreamEndCapture(ctx->stream, &g));
     hipGraphExec_t ge = nullptr;
     RFD_HIP(ctx, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
@@ -148,18 +172,24 @@ rfd_status get_graph(rfd_batch *b, const GraphKey &key, const rfd::BatchArgs &a,
 
 // One full run of the batch with a given tile height; fills host states.
 rfd_status run_once(rfd_batch *b, const rfd_opts &o, int rpl, int prune,
-                    std::vector<rfd::BatchState> &hst, unsigned long long &t0, float &ms) {
+                    std::vector<rfd::BatchState> &hst, unsigned long long &t0, float &ms,
+                    rfd_mlp *net = nullptr, double tau = 0.0, double delta = 0.0) {
     rfd_ctx *ctx = b->ctx;
     rfd::BatchArgs a = make_args(b, rpl, o, prune);
+    if (net) {
+        rfd_status s = rfd_host::mlp_reserve(net, (int)b->npairs);
+        if (s != RFD_OK) return s;
+    }
     RFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (o.use_graph) {
         hipGraphExec_t ge;
-        GraphKey key{rpl, o.stop_mode, prune, o.vefd_gate, kDecisionRounds, 1, a.max_depth, o.eps};
-        rfd_status s = get_graph(b, key, a, &ge);
+        GraphKey key{rpl, o.stop_mode, prune, o.vefd_gate, kDecisionRounds, 1, a.max_depth, o.eps,
+                     net, tau, delta};
+        rfd_status s = get_graph(b, key, a, &ge, net);
         if (s != RFD_OK) return s;
         RFD_HIP(ctx, hipGraphLaunch(ge, ctx->stream));
     } else {
-        enqueue_rounds(b, a, true, kDecisionRounds);
+        enqueue_rounds(b, a, true, kDecisionRounds, net, tau, delta);
         RFD_HIP(ctx, hipGetLastError());
     }
     hst.resize((size_t)b->npairs);
@@ -176,7 +206,8 @@ rfd_status run_once(rfd_batch *b, const rfd_opts &o, int rpl, int prune,
         }
         if (o.use_graph) {
             hipGraphExec_t ge;
-            GraphKey key{rpl, o.stop_mode, prune, o.vefd_gate, kContinueRounds, 0, a.max_depth, o.eps};
+            GraphKey key{rpl, o.stop_mode, prune, o.vefd_gate, kContinueRounds, 0, a.max_depth, o.eps,
+                         nullptr, 0.0, 0.0};
             rfd_status s = get_graph(b, key, a, &ge);
             if (s != RFD_OK) return s;
             RFD_HIP(ctx, hipGraphLaunch(ge, ctx->stream));
@@ -214,6 +245,9 @@ rfd_status rfd_batch_destroy(rfd_batch *b) {
     (void)hipFree(b->count);
     (void)hipFree(b->evals);
     (void)hipFree(b->t0);
+    (void)hipFree(b->feat);
+    (void)hipFree(b->xin);
+    (void)hipFree(b->pred);
     delete b;
     return RFD_OK;
 }
@@ -298,6 +332,9 @@ rfd_status rfd_batch_create(rfd_ctx *ctx, int64_t npairs, const double *const *P
     if ((e = hipMalloc(&b->count, sizeof(int) * 2)) != hipSuccess) return fail(e, "hipMalloc(count)");
     if ((e = hipMalloc(&b->evals, sizeof(unsigned long long))) != hipSuccess) return fail(e, "hipMalloc(evals)");
     if ((e = hipMalloc(&b->t0, sizeof(unsigned long long))) != hipSuccess) return fail(e, "hipMalloc(t0)");
+    if ((e = hipMalloc(&b->feat, sizeof(float) * npairs * rfd::MLP_FEAT_ROW)) != hipSuccess) return fail(e, "hipMalloc(features)");
+    if ((e = hipMalloc(&b->xin, sizeof(uint16_t) * npairs * rfd::MLP_IN_ROW)) != hipSuccess) return fail(e, "hipMalloc(net input)");
+    if ((e = hipMalloc(&b->pred, sizeof(float) * npairs)) != hipSuccess) return fail(e, "hipMalloc(predictions)");
     if ((e = hipMemcpy(b->pairs, b->hpairs.data(), sizeof(rfd::BatchPair) * npairs, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "hipMemcpy(pairs)");
     if ((e = hipMemcpy(b->d_order, b->order.data(), sizeof(int) * npairs, hipMemcpyHostToDevice)) != hipSuccess)
@@ -346,7 +383,9 @@ rfd_status rfd_batch_tune(rfd_batch *b, const rfd_opts *opts, int32_t *rows_per_
     return RFD_OK;
 }
 
-rfd_status rfd_batch_run(rfd_batch *b, const rfd_opts *opts, rfd_result *results, double *latency_ms) {
+static rfd_status batch_run_impl(rfd_batch *b, const rfd_opts *opts, rfd_result *results,
+                                 double *latency_ms, rfd_mlp *net, double tau, double delta,
+                                 int32_t *certified) {
     if (!b || !results) return RFD_ERR_ARG;
     rfd_opts o;
     if (opts) o = *opts; else rfd_default_opts(&o);
@@ -364,7 +403,7 @@ rfd_status rfd_batch_run(rfd_batch *b, const rfd_opts *opts, rfd_result *results
     unsigned long long t0 = 0;
     float ms = 0;
     unsigned long long ev0 = 0;
-    rfd_status s = run_once(b, o, rpl, resolve_prune(o), hst, t0, ms);
+    rfd_status s = run_once(b, o, rpl, resolve_prune(o), hst, t0, ms, net, tau, delta);
     if (s != RFD_OK) return s;
     RFD_HIP(ctx, hipMemcpy(&ev0, b->evals, sizeof(ev0), hipMemcpyDeviceToHost));
     double cells = 0;
@@ -383,6 +422,7 @@ rfd_status rfd_batch_run(rfd_batch *b, const rfd_opts *opts, rfd_result *results
         r.iterations = st.iterations;
         r.ms_total = st.t_done >= t0 ? (double)(st.t_done - t0) / ctx->wall_khz : 0.0;
         if (latency_ms) latency_ms[k] = r.ms_total;
+        if (certified) certified[k] = st.certified;
         const double inner = (double)(p.m - 1) * (double)(p.n - 1);
         const bool vefd_ran = p.m >= 2 && p.n >= 2 && (!o.vefd_gate || p.big);
         cells += (double)p.m * (double)p.n + (vefd_ran ? inner : 0.0) + inner * (double)st.iterations;
@@ -391,6 +431,32 @@ rfd_status rfd_batch_run(rfd_batch *b, const rfd_opts *opts, rfd_result *results
     ctx->stats.ms[RFD_K_BATCH] += ms;
     ctx->stats.cells[RFD_K_BATCH] += cells;
     ctx->stats.evaluated[RFD_K_BATCH] += (double)ev0;
+    return RFD_OK;
+}
+
+rfd_status rfd_batch_run(rfd_batch *b, const rfd_opts *opts, rfd_result *results, double *latency_ms) {
+    return batch_run_impl(b, opts, results, latency_ms, nullptr, 0.0, 0.0, nullptr);
+}
+
+rfd_status rfd_batch_run_learned(rfd_batch *b, rfd_mlp *net, const rfd_opts *opts,
+                                 rfd_result *results, double *latency_ms, int32_t *certified) {
+    if (!b || !net || net->ctx != b->ctx) return RFD_ERR_ARG;
+    return batch_run_impl(b, opts, results, latency_ms, net, net->tau, net->delta, certified);
+}
+
+rfd_status rfd_batch_features(rfd_batch *b, float *out) {
+    if (!b || !out) return RFD_ERR_ARG;
+    rfd_ctx *ctx = b->ctx;
+    RFD_HIP(ctx, hipSetDevice(ctx->device));
+    rfd_opts o;
+    rfd_default_opts(&o);
+    rfd::BatchArgs a = make_args(b, 1, o, 0);
+    hipLaunchKernelGGL(rfd::batch_features, dim3((unsigned)b->npairs), dim3(128), 0, ctx->stream, a,
+                       b->feat, b->xin);
+    RFD_HIP(ctx, hipGetLastError());
+    RFD_HIP(ctx, hipMemcpyAsync(out, b->feat, sizeof(float) * b->npairs * rfd::MLP_FEAT_ROW,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return RFD_OK;
 }
 
