@@ -65,11 +65,14 @@ def cpu_baseline(P, Q, mix: dict, dims: int, budget_s: float = 12.0) -> dict:
 
 def batch_section(count: int, ctx) -> dict:
     """Pairs/s of the batched pipeline on `count` pairs (lengths log-uniform in
-    [500, 20000] per curve, dims 2/3 at random, N(0,1) random walks, seed 0),
-    for sequential bisection and for speculative bisection (same results)."""
+    [500, 20000] per curve, dims 2/3 at random, N(0,1) random walks, seed 0;
+    the network never saw these pairs): plain sequential bisection,
+    speculative bisection (same results), and the learned bracket with either
+    (results within 1e-9 relative of plain)."""
     import numpy as np
     from robotfd_amd import workloads as W
     from robotfd_amd.batch import Batch
+    from robotfd_amd.learned import DEFAULT_WEIGHTS, BracketNet, run_learned
 
     pairs = W.batch_pairs(count, seed=0)
     res = {"workload": f"{count} pairs, lengths log-uniform [500, 20000] per curve, "
@@ -79,20 +82,41 @@ def batch_section(count: int, ctx) -> dict:
         res["rows_per_lane"] = best
         res["tune_ms"] = tune_ms
         ref = None
-        for name, depth in (("plain", 1), ("speculative", 0)):
-            b.run(spec_depth=depth)  # warm
+        net = None
+        if os.path.exists(DEFAULT_WEIGHTS):
+            net = BracketNet(ctx=ctx, path=DEFAULT_WEIGHTS)
+        runs = [("plain", 1, None), ("speculative", 0, None)]
+        if net is not None:
+            runs += [("learned", 1, net), ("learned_speculative", 0, net)]
+        for name, depth, nn in runs:
+            go = (lambda: run_learned(b, nn, spec_depth=depth)) if nn else (lambda: b.run(spec_depth=depth))
+            go()  # warm (graph capture)
             t0 = time.perf_counter()
-            r = b.run(spec_depth=depth)
+            r = go()
             dt = time.perf_counter() - t0
             if ref is None:
                 ref = r
             lat = r["latency_ms"]
-            res[name] = {"pairs_per_s": count / dt, "ms": dt * 1e3,
-                         "p50_latency_ms": float(np.percentile(lat, 50)),
-                         "p99_latency_ms": float(np.percentile(lat, 99)),
-                         "bisected_pairs": int((r["iterations"] > 0).sum()),
-                         "iterations": int(r["iterations"].sum()),
-                         "identical_to_plain": bool(np.array_equal(r["distance"], ref["distance"]))}
+            open_pairs = ref["iterations"] > 0
+            entry = {"pairs_per_s": count / dt, "ms": dt * 1e3,
+                     "p50_latency_ms": float(np.percentile(lat, 50)),
+                     "p99_latency_ms": float(np.percentile(lat, 99)),
+                     "decision_sweeps": int(r["iterations"].sum()),
+                     "max_rel_diff_vs_plain": float(np.max(np.abs(r["distance"] - ref["distance"]) /
+                                                           np.maximum(ref["distance"], 1e-300)))}
+            if nn is not None:
+                entry["certified_frac_of_open"] = float(r["certified"][open_pairs].mean()) if open_pairs.any() else 0.0
+                entry["fallback_frac_of_open"] = 1.0 - entry["certified_frac_of_open"]
+            else:
+                entry["bisected_pairs"] = int(open_pairs.sum())
+            res[name] = entry
+        for name in ("speculative", "learned", "learned_speculative"):
+            if name in res:
+                res[name]["speedup_vs_plain"] = res[name]["pairs_per_s"] / res["plain"]["pairs_per_s"]
+        if net is not None:
+            res["learned_policy"] = {"tau": net.policy[0], "delta": net.policy[1],
+                                     "weights": os.path.relpath(DEFAULT_WEIGHTS, ROOT)}
+            net.close()
     return res
 
 
