@@ -48,7 +48,7 @@ typedef struct {
     int32_t vefd_gate;     /* 1: run VE-FD only when m*n > 10000 (.cu:1343); 0: always */
     int32_t verbose;       /* 1: print the reference's progress lines (frechet.cpp:21-71) */
     double mixed_switch;   /* MIXED: relative bracket width below which FP64 takes over */
-    int32_t rows_per_lane; /* sweep tile height per lane: 0 = library choice, else 1, 2 or 4 */
+    int32_t rows_per_lane; /* sweep tile height per lane: 0 = library choice, else 1..4 */
     int32_t use_graph;     /* batch: 1 = replay the captured hipGraph of all rounds */
     int32_t prune;         /* VE-FD cells above DFD^2 skipped: -1 = library default, 0, 1 */
     int32_t spec_depth;    /* batch: levels of the bisection tree one decision round may
@@ -143,8 +143,8 @@ rfd_status rfd_batch_create(rfd_ctx *ctx, int64_t npairs, const double *const *P
  * round that finished that pair. */
 rfd_status rfd_batch_run(rfd_batch *b, const rfd_opts *opts, rfd_result *results,
                          double *latency_ms);
-/* Times the batch once for each rows-per-lane choice (1, 2, 4), keeps the
- * fastest for later runs and returns it; ms (may be NULL) gets 3 timings. */
+/* Times the batch once for each rows-per-lane choice (1..4), keeps the
+ * fastest for later runs and returns it; ms (may be NULL) gets 4 timings. */
 rfd_status rfd_batch_tune(rfd_batch *b, const rfd_opts *opts, int32_t *rows_per_lane, double *ms);
 rfd_status rfd_batch_destroy(rfd_batch *b);
 

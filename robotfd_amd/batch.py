@@ -45,10 +45,10 @@ class Batch:
         self._keep = []  # the library copied the points
 
     def tune(self, **kw):
-        """Times rows-per-lane 1, 2, 4 on this batch; returns (best, [ms1, ms2, ms4])."""
+        """Times rows-per-lane 1..4 on this batch; returns (best, [ms1, ms2, ms3, ms4])."""
         o = _opts(**kw)
         r = ctypes.c_int32(0)
-        ms = (ctypes.c_double * 3)()
+        ms = (ctypes.c_double * 4)()
         check(lib().rfd_batch_tune(self.handle, ctypes.byref(o), ctypes.byref(r), ms),
               self.ctx.handle)
         return r.value, list(ms)

@@ -73,6 +73,11 @@ def build(verbose: bool = False) -> str:
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
+    # Copy kernel of known size that calibrates rocprofv3's byte counters.
+    probe = os.path.join(CSRC, "copy_probe.hip")
+    if os.path.exists(probe):
+        cmd = [hipcc(), "--offload-arch=gfx950", "-O3", probe, "-o", os.path.join(PKG, "copy_probe")]
+        subprocess.run(cmd, check=True)
     return LIB
 
 

@@ -79,6 +79,8 @@ template <int OP>
 void launch_batch_sweep(int rpl, int grid, hipStream_t st, const rfd::BatchArgs &a) {
     if (rpl == 4)
         hipLaunchKernelGGL((rfd::batch_sweep<OP, 4>), dim3(grid), dim3(64), 0, st, a);
+    else if (rpl == 3)
+        hipLaunchKernelGGL((rfd::batch_sweep<OP, 3>), dim3(grid), dim3(64), 0, st, a);
     else if (rpl == 2)
         hipLaunchKernelGGL((rfd::batch_sweep<OP, 2>), dim3(grid), dim3(64), 0, st, a);
     else
@@ -356,11 +358,11 @@ rfd_status rfd_batch_tune(rfd_batch *b, const rfd_opts *opts, int32_t *rows_per_
     if (opts) o = *opts; else rfd_default_opts(&o);
     rfd_ctx *ctx = b->ctx;
     RFD_HIP(ctx, hipSetDevice(ctx->device));
-    const int cand[3] = {1, 2, 4};
+    const int cand[4] = {1, 2, 3, 4};
     double best = INFINITY;
     int best_r = 4;
     std::vector<rfd::BatchState> hst;
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < 4; ++c) {
         unsigned long long t0 = 0;
         float t = 0;
         // Warm (graph instantiation, first-touch) then time.
@@ -392,7 +394,7 @@ static rfd_status batch_run_impl(rfd_batch *b, const rfd_opts *opts, rfd_result 
     rfd_ctx *ctx = b->ctx;
     RFD_HIP(ctx, hipSetDevice(ctx->device));
     int rpl = o.rows_per_lane;
-    if (rpl != 1 && rpl != 2 && rpl != 4) {
+    if (rpl < 1 || rpl > 4) {
         if (!b->tuned_rpl) {
             rfd_status s = rfd_batch_tune(b, &o, nullptr, nullptr);
             if (s != RFD_OK) return s;

@@ -80,7 +80,7 @@ void launch_sweep_r(int dims, int grid, hipStream_t st, const rfd::SweepArgs &a)
 }
 
 // Rows per lane the kernels are instantiated for.
-bool valid_rows_per_lane(int r) { return r == 1 || r == 2 || r == 4; }
+bool valid_rows_per_lane(int r) { return r >= 1 && r <= 4; }
 
 template <int OP, typename T>
 hipError_t launch_sweep(int dims, int grid, int sync, int rpl, hipStream_t st,
@@ -89,6 +89,8 @@ hipError_t launch_sweep(int dims, int grid, int sync, int rpl, hipStream_t st,
         launch_sweep_r<OP, T, rfd::SYNC_ACQREL, 1>(dims, grid, st, a);
     } else if (rpl == 4) {
         launch_sweep_r<OP, T, rfd::SYNC_SCOPED, 4>(dims, grid, st, a);
+    } else if (rpl == 3) {
+        launch_sweep_r<OP, T, rfd::SYNC_SCOPED, 3>(dims, grid, st, a);
     } else if (rpl == 2) {
         launch_sweep_r<OP, T, rfd::SYNC_SCOPED, 2>(dims, grid, st, a);
     } else {

@@ -41,7 +41,7 @@ def batch_and_ref(gpu):
     b.close()
 
 
-@pytest.mark.parametrize("rows,depth", [(1, 1), (2, 0), (4, 0), (4, 2), (2, 3)])
+@pytest.mark.parametrize("rows,depth", [(1, 1), (2, 0), (3, 0), (4, 0), (4, 2), (2, 3)])
 def test_batch_matches_oracle_per_pair(batch_and_ref, rows, depth):
     """Also pins speculative bisection: walking the evaluated tree must give
     the sequential trajectory (same iterations, same final bracket)."""
@@ -89,6 +89,6 @@ def test_batch_reference_stop_rule(batch_and_ref):
 def test_batch_autotuner_picks_a_valid_height(batch_and_ref):
     b, _ = batch_and_ref
     best, ms = b.tune()
-    assert best in (1, 2, 4)
+    assert best in (1, 2, 3, 4)
     assert all(t > 0 for t in ms)
-    assert ms[[1, 2, 4].index(best)] == min(ms)
+    assert ms[best - 1] == min(ms)
