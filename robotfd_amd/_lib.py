@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "librfd.so")
+# RFD_LIB names another build of the same library (A/B timing of two builds).
+LIB_PATH = os.environ.get("RFD_LIB") or os.path.join(PKG, "librfd.so")
 
 RFD_OK = 0
 STATUS_NAMES = {0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "no GPU device",

@@ -260,7 +260,7 @@ __global__ __launch_bounds__(1024) void batch_plan(BatchArgs a, int phase) {
     }
 }
 
-template <int OP, int R>
+template <int OP, int R, bool PRUNE = false>
 __global__ __launch_bounds__(64) void batch_sweep(BatchArgs a) {
     __shared__ double qring[4][3][64];
     __shared__ double sring[4][Staged<OP>::value][64];
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(64) void batch_sweep(BatchArgs a) {
         job.nbands = (int)((rows + 64LL * R - 1) / (64LL * R));
         job.param = (OP == OP_DEC) ? st.radius[node] : (OP == OP_VEFD ? p.endpoint_lb : 0.0);
         job.thr = (OP == OP_VEFD) ? st.thr : INFINITY;
-        run_band<OP, 3, double, SYNC_SCOPED, R>(job, band, qring, sring, evals, nullptr);
+        run_band<OP, 3, double, SYNC_SCOPED, R, PRUNE>(job, band, qring, sring, evals, nullptr);
     }
     if (a.evals) atomicAdd(a.evals, evals);
 }

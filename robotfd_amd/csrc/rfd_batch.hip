@@ -75,16 +75,25 @@ struct rfd_batch {
 
 namespace {
 
-template <int OP>
-void launch_batch_sweep(int rpl, int grid, hipStream_t st, const rfd::BatchArgs &a) {
+template <int OP, bool PRUNE>
+void launch_batch_sweep_p(int rpl, int grid, hipStream_t st, const rfd::BatchArgs &a) {
     if (rpl == 4)
-        hipLaunchKernelGGL((rfd::batch_sweep<OP, 4>), dim3(grid), dim3(64), 0, st, a);
+        hipLaunchKernelGGL((rfd::batch_sweep<OP, 4, PRUNE>), dim3(grid), dim3(64), 0, st, a);
     else if (rpl == 3)
-        hipLaunchKernelGGL((rfd::batch_sweep<OP, 3>), dim3(grid), dim3(64), 0, st, a);
+        hipLaunchKernelGGL((rfd::batch_sweep<OP, 3, PRUNE>), dim3(grid), dim3(64), 0, st, a);
     else if (rpl == 2)
-        hipLaunchKernelGGL((rfd::batch_sweep<OP, 2>), dim3(grid), dim3(64), 0, st, a);
+        hipLaunchKernelGGL((rfd::batch_sweep<OP, 2, PRUNE>), dim3(grid), dim3(64), 0, st, a);
     else
-        hipLaunchKernelGGL((rfd::batch_sweep<OP, 1>), dim3(grid), dim3(64), 0, st, a);
+        hipLaunchKernelGGL((rfd::batch_sweep<OP, 1, PRUNE>), dim3(grid), dim3(64), 0, st, a);
+}
+
+// The pruning instantiation serves the first VE-FD round when pruning is on
+// (its per-pair thresholds are then finite); the redo round and every other
+// round run without the per-cell test.
+template <int OP>
+void launch_batch_sweep(int rpl, int grid, hipStream_t st, const rfd::BatchArgs &a, bool prune = false) {
+    if (OP == rfd::OP_VEFD && prune) launch_batch_sweep_p<OP, true>(rpl, grid, st, a);
+    else launch_batch_sweep_p<OP, false>(rpl, grid, st, a);
 }
 
 rfd::BatchArgs make_args(rfd_batch *b, int rpl, const rfd_opts &o, int prune) {
@@ -131,7 +140,7 @@ void enqueue_rounds(rfd_batch *b, const rfd::BatchArgs &a, bool with_bounds, int
         for (int ph : phases) {
             hipLaunchKernelGGL(rfd::batch_plan, dim3(1), dim3(1024), 0, st, a, ph);
             if (ph == rfd::PH_DFD) launch_batch_sweep<rfd::OP_DFD>(rpl, grid, st, a);
-            else launch_batch_sweep<rfd::OP_VEFD>(rpl, grid, st, a);
+            else launch_batch_sweep<rfd::OP_VEFD>(rpl, grid, st, a, ph == rfd::PH_VEFD && a.prune);
             hipLaunchKernelGGL(rfd::batch_update, dim3(pg), dim3(256), 0, st, a, ph);
         }
         if (net) {

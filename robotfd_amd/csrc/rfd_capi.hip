@@ -71,12 +71,22 @@ int grid_for(rfd_ctx *ctx, int64_t nb) {
     return (int)std::max<int64_t>(g, 1);
 }
 
+template <int OP, typename T, int SYNC, int R, bool PRUNE>
+void launch_sweep_rp(int dims, int grid, hipStream_t st, const rfd::SweepArgs &a) {
+    if (dims == 2)
+        hipLaunchKernelGGL((rfd::band_sweep<OP, 2, T, SYNC, R, PRUNE>), dim3(grid), dim3(64), 0, st, a);
+    else
+        hipLaunchKernelGGL((rfd::band_sweep<OP, 3, T, SYNC, R, PRUNE>), dim3(grid), dim3(64), 0, st, a);
+}
+
+// VE-FD with a finite threshold runs the pruning instantiation; every other
+// sweep runs without the per-cell test.
 template <int OP, typename T, int SYNC, int R>
 void launch_sweep_r(int dims, int grid, hipStream_t st, const rfd::SweepArgs &a) {
-    if (dims == 2)
-        hipLaunchKernelGGL((rfd::band_sweep<OP, 2, T, SYNC, R>), dim3(grid), dim3(64), 0, st, a);
+    if (OP == rfd::OP_VEFD && std::isfinite(a.thr))
+        launch_sweep_rp<OP, T, SYNC, R, true>(dims, grid, st, a);
     else
-        hipLaunchKernelGGL((rfd::band_sweep<OP, 3, T, SYNC, R>), dim3(grid), dim3(64), 0, st, a);
+        launch_sweep_rp<OP, T, SYNC, R, false>(dims, grid, st, a);
 }
 
 // Rows per lane the kernels are instantiated for.

@@ -243,7 +243,7 @@ template <int OP> struct Staged { static constexpr int value = (OP == OP_DEC) ? 
 // registers at step 64k and written to LDS at step 64(k+1), so global-memory
 // latency stays off the per-step path.  At step s lane l reads points s-l and
 // s-l+1: chunks k-1, k, k+1 are live, the fourth slot is free.
-template <int OP, int D, typename T, int SYNC, int R>
+template <int OP, int D, typename T, int SYNC, int R, bool PRUNE = false>
 __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4][D][64],
                                 T (&sring)[4][Staged<OP>::value][64], unsigned long long &evals,
                                 unsigned long long *tlog) {
@@ -320,6 +320,11 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
     T *sout = slots + (long long)(band & 1) * cols;
     T bin = T(0), bout = T(0);
     const long long nsteps = cols + 63;
+    // q_j of the lane's current column; every lane starts at column 0.
+    T qcar[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) qcar[k] = qring[0][k][0];
+    bool reload = false;
 
     for (long long s = 0; s < nsteps; ++s) {
         const long long j = s - lane;
@@ -378,6 +383,7 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
                     if (lane == 0) flag_store<SYNC>(&a.progress[band], (int)s);
                     bout = dead_out;  // column s
                 }
+                reload = true;
                 s += 63;
                 continue;
             }
@@ -391,14 +397,19 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
         if (lane == 0) left = bval;
 
         if (j >= 0 && j < cols && i0 < rows) {
-            // q_j (and q_{j+1}) of this lane's column, from the LDS ring.
+            // q_{j+1} from the LDS ring; q_j is last step's q_{j+1}, carried in
+            // registers (re-read from the ring after a skipped chunk).
             const int ra = (int)((j >> 6) & 3), oa = (int)(j & 63);
             const long long jb = j + 1;
             const int rb = (int)((jb >> 6) & 3), ob = (int)(jb & 63);
             T qa[D], qb[D];
+            if (OP == OP_DFD || reload) {
+#pragma unroll
+                for (int k = 0; k < D; ++k) qcar[k] = qring[ra][k][oa];
+            }
 #pragma unroll
             for (int k = 0; k < D; ++k) {
-                qa[k] = qring[ra][k][oa];
+                qa[k] = qcar[k];
                 qb[k] = (OP == OP_DFD) ? qa[k] : qring[rb][k][ob];
             }
             T A4b = T(0), inv2b = T(0);
@@ -425,7 +436,7 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
                 } else if (OP == OP_VEFD) {
                     T in = N::min(left, bot[t]);
                     T Tv = in, Rv = in;
-                    if (in < thr) {
+                    if (!PRUNE || in < thr) {
                         T wr = point_seg2<T, D>(p[t + 1], qa, qb);
                         T wt = point_seg2<T, D>(qb, p[t], p[t + 1]);
                         Tv = N::max(wt, in);
@@ -464,7 +475,10 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
                 }
             }
             out = left;
+#pragma unroll
+            for (int k = 0; k < D; ++k) qcar[k] = qb[k];
         }
+        reload = false;
 
         if (has_succ && s >= 63) {
             const long long col = s - 63;
@@ -486,7 +500,7 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
     }
 }
 
-template <int OP, int D, typename T, int SYNC, int R>
+template <int OP, int D, typename T, int SYNC, int R, bool PRUNE = false>
 __global__ __launch_bounds__(64) void band_sweep(SweepArgs a) {
     __shared__ T qring[4][D][64];
     __shared__ T sring[4][Staged<OP>::value][64];
@@ -499,7 +513,7 @@ __global__ __launch_bounds__(64) void band_sweep(SweepArgs a) {
         if (lane == 0) band = atomicAdd(a.queue, 1);
         band = __builtin_amdgcn_readfirstlane(band);
         if (band >= a.nbands) break;
-        run_band<OP, D, T, SYNC, R>(job, band, qring, sring, evals, a.tlog);
+        run_band<OP, D, T, SYNC, R, PRUNE>(job, band, qring, sring, evals, a.tlog);
     }
     if (a.evals) atomicAdd(a.evals, evals);
 }
