@@ -33,6 +33,22 @@ KERNEL_SYMBOL = {"vefd_f64": "band_sweep<1, 3, double>", "vefd_f32": "band_sweep
                  "dfd": "band_sweep<0, 3, double>"}
 
 
+def measured_traffic(m: int, n: int, kernel: str):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (scaled by the copy-kernel calibration), when
+    they were taken on this workload; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_band_sweep_200k.json")
+    if (m, n) != (200_000, 200_000) or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    tag = {"vefd_f64": "band_sweep<1, 3, double", "dfd": "band_sweep<0, 3, double"}.get(kernel)
+    for name, v in data["kernels"].items():
+        if tag and tag in name:
+            return v["traffic_bytes"]
+    return None
+
+
 def cpu_baseline(P, Q, mix: dict, dims: int, budget_s: float = 12.0) -> dict:
     """Times the CPU oracle (single thread) on row samples of each sweep the
     GPU ran, and prices the GPU's per-kernel cell counts at those rates."""
@@ -169,7 +185,7 @@ def main() -> int:
     flops = FLOPS_PER_CELL[dom](dims) * st["evaluated"] / st["launches"]
     achieved = flops / (avg_ms * 1e-3) / 1e12
     roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[dom], "unit": "TFLOP/s",
-            "frac": achieved / PEAK_TFLOPS[dom], "traffic": None,
+            "frac": achieved / PEAK_TFLOPS[dom], "traffic": measured_traffic(m, n, dom),
             "kernel": KERNEL_SYMBOL[dom], "avg_launch_ms": avg_ms,
             "flops_per_cell": FLOPS_PER_CELL[dom](dims),
             "cells_per_launch": st["cells"] / st["launches"],
