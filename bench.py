@@ -26,11 +26,37 @@ METRIC = "cells/s at 200k×200k pts; batched pairs/s learned vs plain bisection 
 FLOPS_PER_CELL = {"vefd_f64": lambda D: 16 * D - 2, "vefd_f32": lambda D: 16 * D - 2,
                   "decide_f64": lambda D: 10 * D + 18, "decide_f32": lambda D: 10 * D + 18,
                   "dfd": lambda D: 3 * D - 1}
-PEAK_TFLOPS = {"vefd_f64": 78.6, "decide_f64": 78.6, "dfd": 78.6,
+PEAK_TFLOPS = {"vefd_f64": 78.6, "decide_f64": 78.6, "dfd": 78.6, "bounds_f64": 78.6,
                "vefd_f32": 157.3, "decide_f32": 157.3}
-KERNEL_SYMBOL = {"vefd_f64": "band_sweep<1, 3, double>", "vefd_f32": "band_sweep<1, 3, float>",
-                 "decide_f64": "band_sweep<2, 3, double>", "decide_f32": "band_sweep<2, 3, float>",
-                 "dfd": "band_sweep<0, 3, double>"}
+# Instantiation run for each kind on the session path: band_sweep<OP, dims, type,
+# scoped hand-off, rows per lane, no pruning>.
+_R = int(os.environ.get("RFD_ROWS", "4"))
+KERNEL_SYMBOL = {k: f"rfd::band_sweep<{op}, 3, {t}, 1, {_R}, false>" for k, op, t in
+                 (("vefd_f64", 1, "double"), ("vefd_f32", 1, "float"), ("decide_f64", 2, "double"),
+                  ("decide_f32", 2, "float"), ("dfd", 0, "double"))}
+# Fused DFD + VE-FD sweep: bounds_sweep<dims, rows per lane, producer waves>;
+# at 200k rows the library picks 13 rows per lane and 6 producer waves.
+_BCFG = int(os.environ.get("RFD_BOUNDS", "136"))
+KERNEL_SYMBOL["bounds_f64"] = f"rfd::bounds_sweep<3, {_BCFG // 10}, {_BCFG % 10}>"
+
+
+def flops_per_launch(kind: str, dims: int, m: int, n: int, evaluated_per_launch: float) -> float:
+    """Algorithmic flops of one launch: the fused sweep does one DFD cell per
+    point pair and one VE-FD cell per segment pair."""
+    if kind == "bounds_f64":
+        return FLOPS_PER_CELL["dfd"](dims) * m * n + FLOPS_PER_CELL["vefd_f64"](dims) * (m - 1) * (n - 1)
+    return FLOPS_PER_CELL[kind](dims) * evaluated_per_launch
+
+
+def sweep_mix(stats: dict, m: int, n: int) -> dict:
+    """Per-sweep cell counts, the fused sweep counted as its DFD and VE-FD parts."""
+    mix = {k: dict(v) for k, v in stats.items()}
+    b = mix.pop("bounds_f64", None)
+    if b:
+        for k, c in (("dfd", m * n), ("vefd_f64", (m - 1) * (n - 1))):
+            e = mix.setdefault(k, {"launches": 0, "ms": 0.0, "cells": 0.0, "evaluated": 0.0})
+            e["cells"] += c * b["launches"]
+    return mix
 
 
 def measured_traffic(m: int, n: int, kernel: str):
@@ -42,7 +68,8 @@ def measured_traffic(m: int, n: int, kernel: str):
         return None
     with open(path) as f:
         data = json.load(f)
-    tag = {"vefd_f64": "band_sweep<1, 3, double", "dfd": "band_sweep<0, 3, double"}.get(kernel)
+    tag = {"vefd_f64": "band_sweep<1, 3, double", "dfd": "band_sweep<0, 3, double",
+           "bounds_f64": KERNEL_SYMBOL["bounds_f64"].replace("rfd::", "")}.get(kernel)
     for name, v in data["kernels"].items():
         if tag and tag in name:
             return v["traffic_bytes"]
@@ -177,17 +204,17 @@ def main() -> int:
     cells = sum(v["cells"] for v in stats.values())
     value = cells / wall
 
-    dom = max((k for k in stats if k in FLOPS_PER_CELL), key=lambda k: stats[k]["ms"])
+    dom = max((k for k in stats if k in PEAK_TFLOPS), key=lambda k: stats[k]["ms"])
     st = stats[dom]
     avg_ms = st["ms"] / st["launches"]
     # Flops of the cells the kernel actually evaluated (pruned / unreachable
     # cells are skipped and not counted).
-    flops = FLOPS_PER_CELL[dom](dims) * st["evaluated"] / st["launches"]
+    flops = flops_per_launch(dom, dims, m, n, st["evaluated"] / st["launches"])
     achieved = flops / (avg_ms * 1e-3) / 1e12
     roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[dom], "unit": "TFLOP/s",
             "frac": achieved / PEAK_TFLOPS[dom], "traffic": measured_traffic(m, n, dom),
             "kernel": KERNEL_SYMBOL[dom], "avg_launch_ms": avg_ms,
-            "flops_per_cell": FLOPS_PER_CELL[dom](dims),
+            "flops_per_launch": flops,
             "cells_per_launch": st["cells"] / st["launches"],
             "evaluated_cells_per_launch": st["evaluated"] / st["launches"]}
 
@@ -211,7 +238,7 @@ def main() -> int:
     if args.batch_pairs:
         out["batch"] = batch_section(args.batch_pairs, ctx)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(P, Q, stats, dims)
+        out["cpu_baseline"] = cpu_baseline(P, Q, sweep_mix(stats, m, n), dims)
     print(json.dumps(out), flush=True)
     return 0
 

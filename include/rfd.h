@@ -70,16 +70,17 @@ typedef struct {
 
 /* Per-kernel timing accumulated on a context's stream (HIP events). */
 typedef struct {
-    int64_t launches[8];   /* index: rfd_kernel_kind */
-    double ms[8];
-    double cells[8];       /* grid cells swept (the algorithmic count) */
-    double evaluated[8];   /* cells whose weights / free intervals were computed;
+    int64_t launches[12];  /* index: rfd_kernel_kind */
+    double ms[12];
+    double cells[12];      /* grid cells swept (the algorithmic count) */
+    double evaluated[12];  /* cells whose weights / free intervals were computed;
                               the rest were skipped as unable to change the result */
 } rfd_stats;
 
 typedef enum {
     RFD_K_DFD = 0, RFD_K_VEFD64 = 1, RFD_K_DEC64 = 2, RFD_K_VEFD32 = 3, RFD_K_DEC32 = 4,
-    RFD_K_BATCH = 5, RFD_K_MLP = 6, RFD_K_OTHER = 7
+    RFD_K_BATCH = 5, RFD_K_MLP = 6, RFD_K_OTHER = 7,
+    RFD_K_BOUNDS64 = 8     /* fused DFD + VE-FD sweep (FP64); cells = m*n + (m-1)*(n-1) */
 } rfd_kernel_kind;
 
 typedef struct rfd_ctx rfd_ctx;

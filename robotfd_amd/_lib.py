@@ -21,8 +21,10 @@ STATUS_NAMES = {0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "no G
 STOP_ABS, STOP_REL = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 
-K_DFD, K_VEFD64, K_DEC64, K_VEFD32, K_DEC32, K_BATCH, K_MLP, K_OTHER = range(8)
-KERNEL_NAMES = ["dfd", "vefd_f64", "decide_f64", "vefd_f32", "decide_f32", "batch", "mlp", "other"]
+K_DFD, K_VEFD64, K_DEC64, K_VEFD32, K_DEC32, K_BATCH, K_MLP, K_OTHER, K_BOUNDS64 = range(9)
+KERNEL_NAMES = ["dfd", "vefd_f64", "decide_f64", "vefd_f32", "decide_f32", "batch", "mlp", "other",
+                "bounds_f64", "kind9", "kind10", "kind11"]
+N_KINDS = 12
 
 
 class Opts(ctypes.Structure):
@@ -46,14 +48,14 @@ class Result(ctypes.Structure):
 
 
 class Stats(ctypes.Structure):
-    _fields_ = [("launches", ctypes.c_int64 * 8), ("ms", ctypes.c_double * 8),
-                ("cells", ctypes.c_double * 8), ("evaluated", ctypes.c_double * 8)]
+    _fields_ = [("launches", ctypes.c_int64 * N_KINDS), ("ms", ctypes.c_double * N_KINDS),
+                ("cells", ctypes.c_double * N_KINDS), ("evaluated", ctypes.c_double * N_KINDS)]
 
     def as_dict(self) -> dict:
         return {KERNEL_NAMES[k]: dict(launches=int(self.launches[k]), ms=float(self.ms[k]),
                                       cells=float(self.cells[k]),
                                       evaluated=float(self.evaluated[k]))
-                for k in range(8) if self.launches[k]}
+                for k in range(N_KINDS) if self.launches[k]}
 
 
 # Every symbol include/rfd.h declares, with its ctypes signature.

@@ -1,0 +1,513 @@
+// Fused bound sweep: DFD (upper bound) and VE-FD (lower bound) of one pair in
+// a single pass over the m x n point grid, with the cell weights computed off
+// the serial min/max chain.
+//
+// Why: in band_sweep one wave both computes a cell's weights (two point-to-
+// segment distances with a divide each for VE-FD) and runs the min/max
+// recursion, so a band advances at one wave's latency-bound pace, bands wait
+// for each other in a chain, and at 200k x 200k a third of the run is the
+// pipeline filling and draining.  Here a workgroup owns a band and splits the
+// work: producer waves compute the weights of the band's next steps into an
+// LDS ring (they depend on the points only, never on the recursion), and one
+// chain wave runs the recursion, a few min/max per cell.  A band then advances
+// at the rate its CU can compute weights, and the chain's own latency is off
+// the critical path, so the fill is short and every SIMD has work.
+//
+// Grid and cells.  DFD cell (i, j) couples p_i and q_j (DFD<D>, .cu:458-641).
+// VE-FD cell (i', j') couples segments p_i'p_i'+1 and q_j'q_j'+1 (VEFD<D>,
+// .cu:200-432); it is evaluated at DFD cell (i' + 1, j' + 1), whose two
+// recursion inputs (the cells below and to the left) are exactly VE-FD's.  Row
+// 0 and column 0 carry VE-FD's boundary: weights there are set so that the
+// uniform min/max formula reproduces the boundary values (row 0 passes +big
+// upward; column 0 leaves row 1 holding r^2 and every other row +big).  Rows
+// past m - 1 (padding of the last band) get weights that pass VE-FD values
+// through unchanged.
+//
+// Layout.  Band b holds rows 64Rb .. 64R(b+1)-1; lane l of the chain wave owns
+// rows 64Rb + Rl + t, t < R, and at step s evaluates column s - l for all of
+// them (the anti-diagonal of band_sweep).  Producer wave w computes, for every
+// lane and step, the weights of rows t = w, w + NPW, ... (< R) into slot s mod KS
+// of the ring: three doubles per cell (DFD distance^2, VE-FD right and top
+// weights), lane-contiguous.  Producers take q_{s-l} from lane l-1's previous
+// point by a DPP shift, with lane 0 fed from a register queue of the next 64
+// points of Q (one coalesced load per 64 steps, issued a chunk ahead).
+//
+// Hand-offs.  Inside the workgroup: producer w publishes the count of steps it
+// has written (LDS word, after s_waitcnt lgkmcnt(0)); the chain publishes the
+// count of steps it has consumed; a producer writes step s only once the chain
+// has consumed step s - KS.  Between bands: as band_sweep (scoped atomics,
+// in-order band queue), with two values per column (DFD, VE-FD), but carried
+// by a third role, the courier wave, through two LDS rings, so that the chain
+// never waits on global memory.
+#pragma once
+
+#include "sweep_kernels.hpp"
+
+namespace rfd {
+
+struct BoundsArgs {
+    const double *P, *Q;   // AoS points (m and n)
+    double *slots;         // [2 buffers][2 kinds: DFD, VE-FD][cols]
+    int *progress;         // per band: columns published
+    int *queue;            // next band to hand out
+    double *result;        // [0] DFD, [1] DFD^2, [2] VE-FD, [3] VE-FD^2
+    long long rows, cols;  // m, n
+    int nbands;
+    double r2;             // endpoint bound squared: VE-FD's entry value
+    unsigned long long *tlog;  // optional, 4 per band (as band_sweep)
+};
+
+// Workgroup hand-off words in LDS.  Relaxed atomics (not volatile) so that,
+// once inlined, they lower to ds_read/ds_write: a generic (flat) access would
+// also wait on this wave's outstanding global-memory operations.
+__device__ inline int lds_get(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
+__device__ inline void lds_put(int *p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
+
+// Ring depth in steps, 1536 R bytes each: about 48 KB per workgroup for up to
+// 8 rows per lane (several workgroups per CU), about 120 KB above that (one
+// workgroup per CU).
+template <int R> struct BoundsRing {
+    static constexpr int budget = (R <= 8) ? 49152 : 122880;
+    static constexpr int KS = (budget / (1536 * R) < 16) ? budget / (1536 * R) : 16;
+    static_assert(KS >= 2, "ring needs two slots");
+};
+
+__device__ inline double dpp_f64(double old, double src, int ctrl_shl) {
+    unsigned long long o = (unsigned long long)__double_as_longlong(old);
+    unsigned long long v = (unsigned long long)__double_as_longlong(src);
+    int lo, hi;
+    if (ctrl_shl) {  // wave_shl:1, lane l takes lane l+1; lane 63 keeps `old`
+        lo = __builtin_amdgcn_update_dpp((int)(unsigned)(o & 0xffffffffull), (int)(unsigned)(v & 0xffffffffull),
+                                         0x130, 0xf, 0xf, false);
+        hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(v >> 32), 0x130, 0xf, 0xf, false);
+    } else {         // wave_shr:1, lane l takes lane l-1; lane 0 keeps `old`
+        lo = __builtin_amdgcn_update_dpp((int)(unsigned)(o & 0xffffffffull), (int)(unsigned)(v & 0xffffffffull),
+                                         0x138, 0xf, 0xf, false);
+        hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(v >> 32), 0x138, 0xf, 0xf, false);
+    }
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                            (unsigned long long)(unsigned)lo));
+}
+
+// Squared distance of x to segment [a, b] given ab = b - a and |ab|^2; the
+// operation order of point_seg2 (.cu:375-394), with the segment terms hoisted.
+template <int D>
+__device__ inline double seg_w(const double *x, const double *a, const double *ab, double ab2) {
+    double apab = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) apab = __builtin_fma(x[k] - a[k], ab[k], apab);
+    double t = apab / ab2;
+    t = __builtin_fmin(1.0, __builtin_fmax(0.0, t));
+    double w = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double d = x[k] - __builtin_fma(t, ab[k], a[k]);
+        w = __builtin_fma(d, d, w);
+    }
+    return w;
+}
+
+// min / max of doubles that are never NaN: a bare v_min_f64 / v_max_f64.
+// __builtin_fmin would first quiet each operand the compiler cannot prove
+// canonical (LDS loads, DPP results, loop-carried values), one extra
+// instruction per operand on the chain's serial path.
+__device__ inline double dmin(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ inline double dmax(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+// R cells of one chain lane at one step.  All 3R weights are read before the
+// first min/max so that one LDS round trip, not R of them, sits on the chain.
+//
+// Per row, with dl / vl the values from the row below and dbot / vbot the
+// row's own values from the previous column:
+//   DFD   (.cu:458-641)  top = max(d, min(dl, dbot)),  dl' = min(top, dbot)
+//   VE-FD (.cu:200-432)  in = min(vl, vbot),  T = max(wt, in),  vl' = R = max(wr, in)
+template <int R>
+__device__ inline void chain_cells(const double *w, double (&dbot)[R], double (&vbot)[R], double &dl,
+                                   double &vl) {
+    double wv[R][3];
+#pragma unroll
+    for (int t = 0; t < R; ++t)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) wv[t][c] = w[(t * 3 + c) * 64];
+#pragma unroll
+    for (int t = 0; t < R; ++t)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) asm volatile("" ::"v"(wv[t][c]));
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        const double d = wv[t][0], wr = wv[t][1], wt = wv[t][2];
+        const double top = dmax(d, dmin(dl, dbot[t]));
+        dl = dmin(top, dbot[t]);
+        dbot[t] = top;
+        const double vin = dmin(vl, vbot[t]);
+        vbot[t] = dmax(wt, vin);
+        vl = dmax(wr, vin);
+    }
+}
+
+// Columns of the band-boundary rings in LDS (incoming and outgoing).
+constexpr int kRing = 128;
+
+// LDS words shared by the waves of one workgroup.
+struct BoundsSync {
+    int *prod;       // [NPW] steps each producer has written
+    int *cons;       // steps the chain has consumed
+    int *in_avail;   // incoming columns in the ring
+    int *out_avail;  // outgoing columns the chain has written to the ring
+    int *out_done;   // outgoing columns the courier has published
+};
+
+template <int R, int KS>
+__device__ inline void bounds_chain(const BoundsArgs &a, const int band, const double *wring,
+                                    const double *inring, double *outring, const BoundsSync &sy,
+                                    const int npw) {
+    const int lane = (int)(threadIdx.x & 63);
+    const long long rows = a.rows, cols = a.cols;
+    const double big = (double)FLT_MAX;
+    const long long i0 = (long long)band * 64 * R + (long long)R * lane;
+    double dbot[R], vbot[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        dbot[t] = big;
+        vbot[t] = -big;  // column 0's weights then set the row's VE-FD entry value
+    }
+    double dout = big, vout = big;
+    const bool has_pred = band > 0;
+    const bool has_succ = band + 1 < a.nbands;
+    const long long nsteps = cols + 63;
+    const long long tl = rows - 1 - i0;  // the terminal row's offset in this lane (if in [0, R))
+    long long avail = 0, in_known = 0, done_known = 0;
+    int slot = 0;
+    if (a.tlog && lane == 0) a.tlog[4 * band] = wall_clock64();
+    // The chain is the band's serial path and every later band waits on it:
+    // its SIMD issues for it first, producers fill the remaining cycles.
+    __builtin_amdgcn_s_setprio(3);
+
+    for (long long s = 0; s < nsteps; ++s) {
+        const long long j = s - lane;
+        // Row 64Rb - 1 at column s (lane 0's input from below), from the ring
+        // the courier fills.
+        if (has_pred && s < cols && s >= in_known) {
+            for (;;) {
+                in_known = __builtin_amdgcn_readfirstlane(lds_get(sy.in_avail));
+                if (s < in_known) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            if (a.tlog && lane == 0 && s == 0) a.tlog[4 * band + 1] = wall_clock64();
+        }
+        if (s >= avail) {
+            for (;;) {
+                int mn = lds_get(&sy.prod[0]);
+                for (int w = 1; w < npw; ++w) {
+                    const int v = lds_get(&sy.prod[w]);
+                    mn = v < mn ? v : mn;
+                }
+                avail = __builtin_amdgcn_readfirstlane(mn);
+                if (s < avail) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        }
+        const double *w = wring + (long long)slot * (R * 3 * 64) + lane;
+        double dl = shift_up(dout), vl = shift_up(vout);
+        double bD, bV;
+        if (has_pred) {
+            const int r = (int)((s < cols ? s : cols - 1) % kRing);
+            bD = inring[r];
+            bV = inring[kRing + r];
+        } else {
+            bD = (s == 0) ? 0.0 : big;
+            bV = big;
+        }
+        if (lane == 0) {
+            dl = bD;
+            vl = bV;
+        }
+        if (j >= 0 && j < cols) {  // every lane, except while the band fills and drains
+            chain_cells<R>(w, dbot, vbot, dl, vl);
+            dout = dl;
+            vout = vl;
+        }
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (lane == 0) lds_put(sy.cons, (int)(s + 1));
+        slot = (slot + 1 == KS) ? 0 : slot + 1;
+
+        // Row 64R(b+1) - 1 at column s - 63 (lane 63's output) into the
+        // outgoing ring, for the courier to publish.
+        if (has_succ && s >= 63) {
+            const long long col = s - 63;
+            if (col >= done_known + kRing) {
+                for (;;) {
+                    done_known = __builtin_amdgcn_readfirstlane(lds_get(sy.out_done));
+                    if (col < done_known + kRing) break;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            }
+            if (lane == 63) {
+                const int r = (int)(col % kRing);
+                outring[r] = dout;
+                outring[kRing + r] = vout;
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                lds_put(sy.out_avail, (int)(col + 1));
+            }
+        }
+    }
+    __builtin_amdgcn_s_setprio(0);
+    // Terminal cell (m-1, n-1): its DFD value is the row's last top; its VE-FD
+    // value is min(T, R), where R left row tl at the last column and, rows
+    // above tl being padding that passes VE-FD values through, is vout.
+    if (tl >= 0 && tl < R) {
+        double topD = dbot[0], topV = vbot[0];
+#pragma unroll
+        for (int t = 1; t < R; ++t) {
+            topD = (tl == t) ? dbot[t] : topD;
+            topV = (tl == t) ? vbot[t] : topV;
+        }
+        const double resV = dmin(topV, vout);
+        a.result[0] = __builtin_sqrt(topD);
+        a.result[1] = topD;
+        a.result[2] = __builtin_sqrt(resV);
+        a.result[3] = resV;
+    }
+    if (a.tlog && lane == 0) {
+        a.tlog[4 * band + 2] = wall_clock64();
+        a.tlog[4 * band + 3] = ((unsigned long long)__builtin_amdgcn_s_getreg((20 << 0) | (0 << 6) | (31 << 11)) << 32) |
+                               (unsigned long long)__builtin_amdgcn_s_getreg((4 << 0) | (0 << 6) | (31 << 11));
+    }
+}
+
+// The band's global-memory hand-offs, kept off the chain: the courier copies
+// the predecessor's published boundary columns into the incoming ring and
+// publishes the chain's outgoing columns (scoped stores, s_waitcnt, flag), as
+// many columns per round as are ready, up to 64.
+__device__ inline void bounds_courier(const BoundsArgs &a, const int band, double *inring,
+                                      const double *outring, const BoundsSync &sy) {
+    const int lane = (int)(threadIdx.x & 63);
+    const long long cols = a.cols;
+    const double *inD = a.slots + (long long)((band + 1) & 1) * 2 * cols;
+    const double *inV = inD + cols;
+    double *outD = a.slots + (long long)(band & 1) * 2 * cols;
+    double *outV = outD + cols;
+    long long in_next = band > 0 ? 0 : cols;
+    long long out_next = band + 1 < a.nbands ? 0 : cols;
+    while (in_next < cols || out_next < cols) {
+        bool moved = false;
+        if (in_next < cols) {
+            // Ring slots free: the chain has consumed columns below `cons`.
+            const long long room = (long long)__builtin_amdgcn_readfirstlane(lds_get(sy.cons)) + kRing - in_next;
+            if (room > 0) {
+                const long long f = __builtin_amdgcn_readfirstlane(flag_load<SYNC_SCOPED>(&a.progress[band - 1]));
+                long long n = f - in_next;
+                n = n < room ? n : room;
+                n = n < 64 ? n : 64;
+                if (n > 0) {
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    if (lane < n) {
+                        const long long c = in_next + lane;
+                        const double vD = column_load<SYNC_SCOPED>(inD + c);
+                        const double vV = column_load<SYNC_SCOPED>(inV + c);
+                        inring[c % kRing] = vD;
+                        inring[kRing + c % kRing] = vV;
+                    }
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the ring writes are done
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    if (lane == 0) lds_put(sy.in_avail, (int)(in_next + n));
+                    in_next += n;
+                    moved = true;
+                }
+            }
+        }
+        if (out_next < cols) {
+            const long long oa = __builtin_amdgcn_readfirstlane(lds_get(sy.out_avail));
+            long long n = oa - out_next;
+            n = n < 64 ? n : 64;
+            if (n > 0) {
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                if (lane < n) {
+                    const long long c = out_next + lane;
+                    column_store<SYNC_SCOPED>(outD + c, outring[c % kRing]);
+                    column_store<SYNC_SCOPED>(outV + c, outring[kRing + c % kRing]);
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0) flag_store<SYNC_SCOPED>(&a.progress[band], (int)(out_next + n));
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                if (lane == 0) lds_put(sy.out_done, (int)(out_next + n));
+                out_next += n;
+                moved = true;
+            }
+        }
+        if (!moved) __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+template <int D, int R, int NPW, int KS>
+__device__ inline void bounds_producer(const BoundsArgs &a, const int band, const int pw, double *wring,
+                                       int *prod, int *cons) {
+    // Rows t = pw, pw + NPW, ... (< R) of every lane.
+    constexpr int RP = (R + NPW - 1) / NPW;
+    const int nmine = (R - pw + NPW - 1) / NPW;
+    const int lane = (int)(threadIdx.x & 63);
+    const long long rows = a.rows, cols = a.cols;
+    const double big = (double)FLT_MAX;
+    const double *P = a.P, *Q = a.Q;
+    const long long i0 = (long long)band * 64 * R + (long long)R * lane;
+    // Row terms: p_i, p_{i-1}, p_i - p_{i-1} and its squared length.
+    double pi[RP][D], pm[RP][D], ab[RP][D], ab2[RP];
+#pragma unroll
+    for (int k = 0; k < RP; ++k) {
+        const long long i = i0 + pw + (long long)k * NPW;
+        const long long ic = i < rows ? i : rows - 1;
+        long long im = i - 1 < rows ? i - 1 : rows - 1;
+        im = im > 0 ? im : 0;
+        double acc = 0.0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            pi[k][c] = P[ic * D + c];
+            pm[k][c] = P[im * D + c];
+            ab[k][c] = pi[k][c] - pm[k][c];
+            acc = __builtin_fma(ab[k][c], ab[k][c], acc);
+        }
+        ab2[k] = acc;
+    }
+    // Q queue: lane l holds point s0 + l of the current 64-point chunk, shifted
+    // down one lane per step so lane 0 holds q_s at step s.
+    double qq[D], qn[D], qa[D], qb[D];
+    {
+        const long long c0 = lane < cols ? lane : cols - 1;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            qq[k] = Q[c0 * D + k];
+            qn[k] = qq[k];
+            qb[k] = 0.0;
+        }
+    }
+    const double r2 = a.r2;
+    const long long nsteps = cols + 63;
+    long long cons_seen = 0;
+    int slot = 0;
+    for (long long s = 0; s < nsteps; ++s) {
+        if ((s & 63) == 0) {
+            if (s > 0) {
+#pragma unroll
+                for (int k = 0; k < D; ++k) qq[k] = qn[k];
+            }
+            long long c = s + 64 + lane;  // the chunk after this one, used 64 steps later
+            c = c < cols ? c : cols - 1;
+#pragma unroll
+            for (int k = 0; k < D; ++k) qn[k] = Q[c * D + k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < D; ++k) qq[k] = dpp_f64(qq[k], qq[k], 1);
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            qa[k] = qb[k];
+            qb[k] = dpp_f64(qq[k], qb[k], 0);  // q_{s-l}; lane 0 takes q_s from the queue
+        }
+        // Column terms of segment [q_{j-1}, q_j].
+        double cab[D], cab2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            cab[k] = qb[k] - qa[k];
+            cab2 = __builtin_fma(cab[k], cab[k], cab2);
+        }
+        if (s >= cons_seen + KS) {
+            for (;;) {
+                cons_seen = __builtin_amdgcn_readfirstlane(lds_get(cons));
+                if (s < cons_seen + KS) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        }
+        double *w = wring + (long long)slot * (R * 3 * 64) + lane;
+        const bool edge = s < 64 || band == 0 || band + 1 == a.nbands;
+        const long long j = s - lane;
+#pragma unroll
+        for (int k = 0; k < RP; ++k) {
+            if (RP * NPW != R && k == RP - 1 && k >= nmine) break;
+            const int t = pw + k * NPW;
+            double d = 0.0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                const double x = pi[k][c] - qb[c];
+                d = __builtin_fma(x, x, d);
+            }
+            const double wr = seg_w<D>(pi[k], qa, cab, cab2);
+            const double wt = seg_w<D>(qb, pm[k], ab[k], ab2[k]);
+            w[(t * 3 + 0) * 64] = d;
+            w[(t * 3 + 1) * 64] = wr;
+            w[(t * 3 + 2) * 64] = wt;
+        }
+        if (edge) {
+            // Boundary and padding cells overwrite their VE-FD weights.  A real
+            // branch (the asm keeps the compiler from turning it into selects
+            // on every step): it is taken while the band fills, and in the
+            // first and last bands.
+            asm volatile("");
+#pragma unroll
+            for (int k = 0; k < RP; ++k) {
+                if (RP * NPW != R && k == RP - 1 && k >= nmine) break;
+                const int t = pw + k * NPW;
+                const long long i = i0 + t;
+                const bool pad = i >= rows;  // padding rows of the last band pass VE-FD values up unchanged
+                if (j == 0 || i == 0 || pad) {
+                    w[(t * 3 + 1) * 64] = pad ? -big : big;
+                    w[(t * 3 + 2) * 64] = (!pad && j == 0 && i == 1) ? r2 : big;
+                }
+            }
+        }
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this step's weights are in LDS
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (lane == 0) lds_put(&prod[pw], (int)(s + 1));
+        slot = (slot + 1 == KS) ? 0 : slot + 1;
+    }
+}
+
+template <int D, int R, int NPW>
+__global__ __launch_bounds__(64 * (NPW + 2)) void bounds_sweep(BoundsArgs a) {
+    static_assert(NPW >= 1 && NPW <= R, "one to R producer waves");
+    constexpr int KS = BoundsRing<R>::KS;
+    __shared__ double wring[KS * R * 3 * 64];
+    __shared__ double inring[2 * kRing];
+    __shared__ double outring[2 * kRing];
+    __shared__ int sprod[NPW];
+    __shared__ int swords[4];  // cons, in_avail, out_avail, out_done
+    __shared__ int sband;
+    const int wave = (int)(threadIdx.x >> 6);
+    const BoundsSync sy{sprod, &swords[0], &swords[1], &swords[2], &swords[3]};
+    for (;;) {
+        if (threadIdx.x == 0) {
+            sband = atomicAdd(a.queue, 1);
+            for (int w = 0; w < NPW; ++w) sprod[w] = 0;
+            for (int k = 0; k < 4; ++k) swords[k] = 0;
+        }
+        __syncthreads();
+        const int band = sband;
+        if (band >= a.nbands) break;
+        // Waves of a workgroup are dealt to the CU's four SIMDs in turn, so
+        // wave 4 shares a SIMD with wave 0: that is the courier, which mostly
+        // sleeps, and the chain's SIMD keeps its issue slots for the chain.
+        constexpr int cw = (NPW + 2 > 4) ? 4 : NPW + 1;
+        if (wave == 0)
+            bounds_chain<R, KS>(a, band, wring, inring, outring, sy, NPW);
+        else if (wave == cw)
+            bounds_courier(a, band, inring, outring, sy);
+        else
+            bounds_producer<D, R, NPW, KS>(a, band, wave - 1 - (wave > cw ? 1 : 0), wring, sprod, sy.cons);
+        __syncthreads();
+    }
+}
+
+}  // namespace rfd

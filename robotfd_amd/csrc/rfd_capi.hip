@@ -7,6 +7,7 @@
 #include "rfd.h"
 #include "rfd_internal.hpp"
 #include "sweep_kernels.hpp"
+#include "bounds_kernels.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -31,7 +32,7 @@ struct rfd_session {
     void *slots = nullptr;
     int *progress = nullptr;
     int *queue = nullptr;
-    double *result = nullptr;            // [0] terminal value, [1] its square (DFD, VE-FD)
+    double *result = nullptr;            // [0] terminal value, [1] its square; fused bounds: [0..3]
     unsigned long long *evals = nullptr; // cells evaluated by the last sweep
     double endpoint_lb = 0.0;
     double dfd_sq = -1.0;                // DFD^2 of this pair once known (prunes VE-FD)
@@ -39,15 +40,14 @@ struct rfd_session {
 
 namespace rfd {
 
-__global__ void sweep_init(int *progress, int nbands, int *queue, double *result, double sentinel,
-                           unsigned long long *evals) {
+__global__ void sweep_init(int *progress, int nbands, int *queue, double *result, int nres,
+                           double sentinel, unsigned long long *evals) {
     const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int stride = (int)(gridDim.x * blockDim.x);
     for (int k = tid; k < nbands; k += stride) progress[k] = 0;
     if (tid == 0) {
         *queue = 0;
-        result[0] = sentinel;
-        result[1] = sentinel;
+        for (int k = 0; k < nres; ++k) result[k] = sentinel;
         *evals = 0;
     }
 }
@@ -173,7 +173,7 @@ rfd_status run_sweep(rfd_session *s, const T *P, const T *Q, const T *qseg, doub
     const int grid = grid_for(ctx, nb);
     RFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     hipLaunchKernelGGL(rfd::sweep_init, dim3((unsigned)std::min<int64_t>((nb + 255) / 256, 1024)),
-                       dim3(256), 0, ctx->stream, s->progress, (int)nb, s->queue, s->result,
+                       dim3(256), 0, ctx->stream, s->progress, (int)nb, s->queue, s->result, 2,
                        sentinel, s->evals);
     RFD_HIP(ctx, hipGetLastError());
     RFD_HIP(ctx, (launch_sweep<OP, T>(s->dims, grid, ctx->sync, rpl, ctx->stream, a)));
@@ -209,6 +209,125 @@ rfd_status run_sweep(rfd_session *s, const T *P, const T *Q, const T *qseg, doub
         return RFD_ERR_STATE;
     }
     *result = h;
+    return RFD_OK;
+}
+
+// Fused DFD + VE-FD sweep.  Configurations: rows per lane R and producer
+// waves NPW (one chain wave besides), encoded as 10 * R + NPW.
+template <int D, int R, int NPW>
+hipError_t launch_bounds_cfg(rfd_ctx *ctx, int64_t nb, const rfd::BoundsArgs &a, int *grid_out) {
+    auto kern = rfd::bounds_sweep<D, R, NPW>;
+    // Optional cap on workgroups per CU (RFD_WGCU): unused dynamic LDS sized so
+    // that one more workgroup does not fit.  Bands on a CU shared by more
+    // workgroups run slower, and every later band waits on the slowest.
+    size_t dyn = 0;
+    if (ctx->wg_per_cu > 0 && ctx->lds_per_cu > 0) {
+        hipFuncAttributes fa;
+        hipError_t e0 = hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(kern));
+        if (e0 != hipSuccess) return e0;
+        const size_t want = (size_t)ctx->lds_per_cu / (size_t)(ctx->wg_per_cu + 1) + 1;
+        if (want > fa.sharedSizeBytes) dyn = want - fa.sharedSizeBytes;
+    }
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * (NPW + 2), dyn);
+    if (e != hipSuccess) return e;
+    int64_t grid = std::min<int64_t>(nb, (int64_t)ctx->cu_count * std::max(per_cu, 1));
+    grid = std::max<int64_t>(1, std::min<int64_t>(grid, ctx->max_waves));  // RFD_MAX_WAVES caps workgroups too
+    *grid_out = (int)grid;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * (NPW + 2)), dyn, ctx->stream, a);
+    return hipGetLastError();
+}
+
+bool valid_bounds_cfg(int cfg) {
+    return cfg == 42 || cfg == 62 || cfg == 63 || cfg == 72 || cfg == 73 || cfg == 83 || cfg == 84 || cfg == 21 ||
+           cfg == 136;
+}
+
+template <int D>
+hipError_t launch_bounds(rfd_ctx *ctx, int cfg, int64_t nb, const rfd::BoundsArgs &a, int *grid) {
+    switch (cfg) {
+        case 21: return launch_bounds_cfg<D, 2, 1>(ctx, nb, a, grid);
+        case 62: return launch_bounds_cfg<D, 6, 2>(ctx, nb, a, grid);
+        case 63: return launch_bounds_cfg<D, 6, 3>(ctx, nb, a, grid);
+        case 72: return launch_bounds_cfg<D, 7, 2>(ctx, nb, a, grid);
+        case 73: return launch_bounds_cfg<D, 7, 3>(ctx, nb, a, grid);
+        case 83: return launch_bounds_cfg<D, 8, 3>(ctx, nb, a, grid);
+        case 84: return launch_bounds_cfg<D, 8, 4>(ctx, nb, a, grid);
+        case 136: return launch_bounds_cfg<D, 13, 6>(ctx, nb, a, grid);
+        default: return launch_bounds_cfg<D, 4, 2>(ctx, nb, a, grid);
+    }
+}
+
+// One fused bound sweep of a session (m, n >= 2): DFD, DFD^2 and VE-FD.
+rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, double *vefd) {
+    rfd_ctx *ctx = s->ctx;
+    // Configuration: RFD_BOUNDS if set; otherwise 13 rows per lane (one band
+    // per CU, 6 producer waves) when that still gives most CUs a band, else 4
+    // rows per lane with 2 producer waves (several bands per CU).
+    int cfg = ctx->bounds_cfg;
+    if (cfg == 0) {
+        const int64_t nb13 = (s->m + 64 * 13 - 1) / (64 * 13);
+        cfg = (4 * nb13 >= 3 * (int64_t)ctx->cu_count) ? 136 : 42;
+    }
+    const int R = cfg / 10;
+    const int64_t rows = s->m, cols = s->n;
+    const int64_t nb = (rows + 64 * R - 1) / (64 * R);
+    rfd::BoundsArgs a{};
+    a.P = s->dP;
+    a.Q = s->dQ;
+    a.slots = (double *)s->slots;
+    a.progress = s->progress;
+    a.queue = s->queue;
+    a.result = s->result;
+    a.rows = rows;
+    a.cols = cols;
+    a.nbands = (int)nb;
+    a.r2 = init * init;
+    const double sentinel = -3.0;
+    unsigned long long *tlog = nullptr;
+    if (ctx->tlog_path) {
+        RFD_HIP(ctx, hipMalloc(&tlog, sizeof(unsigned long long) * 4 * nb));
+        RFD_HIP(ctx, hipMemsetAsync(tlog, 0, sizeof(unsigned long long) * 4 * nb, ctx->stream));
+    }
+    a.tlog = tlog;
+    RFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    hipLaunchKernelGGL(rfd::sweep_init, dim3((unsigned)std::min<int64_t>((nb + 255) / 256, 1024)),
+                       dim3(256), 0, ctx->stream, s->progress, (int)nb, s->queue, s->result, 4,
+                       sentinel, s->evals);
+    RFD_HIP(ctx, hipGetLastError());
+    int grid = 0;
+    RFD_HIP(ctx, s->dims == 2 ? launch_bounds<2>(ctx, cfg, nb, a, &grid) : launch_bounds<3>(ctx, cfg, nb, a, &grid));
+    RFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    double hres[4] = {0, 0, 0, 0};
+    RFD_HIP(ctx, hipMemcpyAsync(hres, s->result, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    RFD_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    // Cells of both bound sweeps: m * n DFD cells and (m-1)(n-1) VE-FD cells.
+    const double cells = (double)rows * (double)cols + (double)(rows - 1) * (double)(cols - 1);
+    ctx->stats.launches[RFD_K_BOUNDS64] += 1;
+    ctx->stats.ms[RFD_K_BOUNDS64] += ms;
+    ctx->stats.cells[RFD_K_BOUNDS64] += cells;
+    ctx->stats.evaluated[RFD_K_BOUNDS64] += cells;
+    if (tlog) {
+        std::vector<unsigned long long> h_t((size_t)(4 * nb));
+        RFD_HIP(ctx, hipMemcpy(h_t.data(), tlog, sizeof(unsigned long long) * 4 * nb, hipMemcpyDeviceToHost));
+        RFD_HIP(ctx, hipFree(tlog));
+        if (FILE *f = std::fopen(ctx->tlog_path, "ab")) {
+            long long hdr[6] = {(long long)RFD_K_BOUNDS64, (long long)rows, (long long)cols, (long long)nb,
+                                (long long)grid, (long long)cfg};
+            std::fwrite(hdr, sizeof(hdr), 1, f);
+            std::fwrite(h_t.data(), sizeof(unsigned long long), h_t.size(), f);
+            std::fclose(f);
+        }
+    }
+    if (hres[0] == sentinel || hres[2] == sentinel) {
+        set_err(ctx, "fused bound sweep wrote no terminal value");
+        return RFD_ERR_STATE;
+    }
+    *dfd = hres[0];
+    *dfd_sq = hres[1];
+    *vefd = hres[2];
     return RFD_OK;
 }
 
@@ -282,6 +401,17 @@ rfd_status rfd_create(rfd_ctx **out, int device) {
     if (const char *e = std::getenv("RFD_SYNC")) ctx->sync = std::atoi(e) == 0 ? rfd::SYNC_ACQREL : rfd::SYNC_SCOPED;
     ctx->tlog_path = std::getenv("RFD_TLOG");
     if (const char *e = std::getenv("RFD_PRUNE")) ctx->prune = std::atoi(e) != 0;
+    if (const char *e = std::getenv("RFD_FUSED")) ctx->fused = std::atoi(e) != 0;
+    if (const char *e = std::getenv("RFD_BOUNDS")) {
+        int c = std::atoi(e);
+        if (valid_bounds_cfg(c)) ctx->bounds_cfg = c;
+    }
+    if (const char *e = std::getenv("RFD_WGCU")) ctx->wg_per_cu = std::max(0, std::atoi(e));
+    {
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) == hipSuccess)
+            ctx->lds_per_cu = lds;
+    }
     if (const char *e = std::getenv("RFD_ROWS")) {
         int r = std::atoi(e);
         if (valid_rows_per_lane(r)) ctx->rows_per_lane = r;
@@ -359,12 +489,13 @@ rfd_status rfd_session_create(rfd_ctx *ctx, const double *P, int64_t m, const do
     hipError_t e;
     if ((e = hipMalloc(&s->dP, sizeof(double) * m * dims)) != hipSuccess) return fail(e, "hipMalloc(P)");
     if ((e = hipMalloc(&s->dQ, sizeof(double) * n * dims)) != hipSuccess) return fail(e, "hipMalloc(Q)");
-    // Band-boundary buffers: two columns of n doubles; progress flags: one per band.
-    if ((e = hipMalloc(&s->slots, sizeof(double) * 2 * n)) != hipSuccess) return fail(e, "hipMalloc(slots)");
+    // Band-boundary buffers: two columns of n doubles (two kinds each for the
+    // fused bound sweep); progress flags: one per band.
+    if ((e = hipMalloc(&s->slots, sizeof(double) * 4 * n)) != hipSuccess) return fail(e, "hipMalloc(slots)");
     if ((e = hipMalloc(&s->progress, sizeof(int) * (nbands_for(m) + 1))) != hipSuccess)
         return fail(e, "hipMalloc(progress)");
     if ((e = hipMalloc(&s->queue, sizeof(int))) != hipSuccess) return fail(e, "hipMalloc(queue)");
-    if ((e = hipMalloc(&s->result, 2 * sizeof(double))) != hipSuccess) return fail(e, "hipMalloc(result)");
+    if ((e = hipMalloc(&s->result, 4 * sizeof(double))) != hipSuccess) return fail(e, "hipMalloc(result)");
     if ((e = hipMalloc(&s->evals, sizeof(unsigned long long))) != hipSuccess) return fail(e, "hipMalloc(evals)");
     if ((e = hipMemcpyAsync(s->dP, P, sizeof(double) * m * dims, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
         return fail(e, "hipMemcpy(P)");
@@ -380,6 +511,11 @@ rfd_status rfd_session_bounds(rfd_session *s, double *dfd, double *vefd, int run
     rfd_ctx *ctx = s->ctx;
     RFD_HIP(ctx, hipSetDevice(ctx->device));
     double F = 0;
+    if (ctx->fused && run_vefd && !fp32 && !ctx->prune && s->m >= 2 && s->n >= 2) {
+        rfd_status st = run_bounds(s, *vefd, dfd, &F, vefd);
+        if (st == RFD_OK) s->dfd_sq = F;
+        return st;
+    }
     rfd_status st = run_sweep<rfd::OP_DFD, double>(s, s->dP, s->dQ, nullptr, 0.0, RFD_K_DFD, dfd,
                                                    INFINITY, &F);
     if (st != RFD_OK) return st;

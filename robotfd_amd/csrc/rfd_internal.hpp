@@ -21,6 +21,10 @@ struct rfd_ctx {
     int sync = rfd::SYNC_SCOPED;  // band hand-off protocol (RFD_SYNC=0 selects acquire/release)
     int rows_per_lane = 4;        // R of band_sweep (RFD_ROWS overrides: 1, 2 or 4)
     bool prune = false;           // VE-FD skips cells above DFD^2 (RFD_PRUNE=1 enables)
+    bool fused = true;            // FP64 bounds by the fused DFD + VE-FD sweep (RFD_FUSED=0 disables)
+    int bounds_cfg = 0;           // fused sweep: 10 * rows per lane + producer waves, 0 = by size (RFD_BOUNDS)
+    int wg_per_cu = 0;            // fused sweep: cap on workgroups per CU, 0 = none (RFD_WGCU)
+    int lds_per_cu = 0;           // bytes of LDS per CU
     const char *tlog_path = nullptr;  // RFD_TLOG: append per-band timestamps of every sweep
     rfd_stats stats{};
     char err[512] = {0};

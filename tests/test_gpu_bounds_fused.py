@@ -1,0 +1,127 @@
+"""The fused bound sweep (DFD and VE-FD in one pass, weights computed by
+producer waves off the min/max chain) against the CPU oracle, bit-exact, for
+every instantiated configuration, workgroups taking several bands each, and
+the separate band sweeps it replaces."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from robotfd_amd import Context, Intersection, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(2, 2), (2, 3), (3, 2), (5, 7), (63, 40), (64, 64), (65, 66), (128, 70), (129, 300),
+          (200, 129), (700, 50), (40, 700), (257, 100), (520, 77), (1100, 90)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def walk(rng, k, d, sigma=1.0):
+    return np.cumsum(rng.standard_normal((k, d)) * sigma, axis=0)
+
+
+def make_ctx(**env):
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update({k: str(v) for k, v in env.items()})
+    try:
+        return Context()
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+
+
+@pytest.fixture(scope="module")
+def contexts():
+    made = {}
+
+    def get(**env):
+        key = tuple(sorted(env.items()))
+        if key not in made:
+            made[key] = make_ctx(**env)
+        return made[key]
+    yield get
+    for c in made.values():
+        c.close()
+
+
+def check_pair(ctx, P, Q):
+    lb = O.endpoint_lb(P, Q)
+    with Intersection(P, Q, ctx=ctx) as s:
+        dfd, ve = s.Bound_call_gpu(lb)
+    assert dfd == O.dfd(P, Q)
+    assert ve == O.vefd(P, Q, lb)
+
+
+@pytest.mark.parametrize("cfg", [21, 42, 62, 63, 72, 73, 83, 84, 136])
+@pytest.mark.parametrize("dims", [2, 3])
+def test_fused_configs_bit_exact(contexts, cfg, dims):
+    ctx = contexts(RFD_BOUNDS=cfg)
+    ctx.reset_stats()
+    for m, n in SHAPES:
+        rng = np.random.default_rng(m * 1000 + n + dims + cfg)
+        P, Q = walk(rng, m, dims), walk(rng, n, dims)
+        say("fused", cfg, dims, (m, n))
+        check_pair(ctx, P, Q)
+    st = ctx.stats()
+    assert st["bounds_f64"]["launches"] == len(SHAPES) and "dfd" not in st
+
+
+@pytest.mark.parametrize("cfg", [42, 84, 136])
+def test_fused_workgroups_take_several_bands(contexts, cfg):
+    ctx = contexts(RFD_BOUNDS=cfg, RFD_MAX_WAVES=3)
+    rng = np.random.default_rng(5)
+    P, Q = walk(rng, 64 * 8 * 7 + 5, 3), walk(rng, 333, 3)
+    say("wrap", cfg)
+    check_pair(ctx, P, Q)
+
+
+def test_fused_noisy_copy_and_collapsed_bracket(contexts):
+    ctx = contexts()
+    for seed in range(3):
+        P, Q = W.robot_pair(900 + 50 * seed, 1000, seed=seed)
+        say("robot", seed)
+        check_pair(ctx, P, Q)
+    P, Q = W.walk_pair(3000, 2500, dims=3, seed=0)
+    say("walk 3000x2500")
+    check_pair(ctx, P, Q)
+
+
+def test_fused_degenerate_segments(contexts):
+    """Repeated points give zero-length segments: the projection divides 0 by 0
+    and the clamp must take it to 0 as the oracle does."""
+    ctx = contexts()
+    rng = np.random.default_rng(8)
+    P, Q = walk(rng, 150, 2), walk(rng, 170, 2)
+    P[40:45] = P[40]
+    Q[::7] = Q[3]
+    check_pair(ctx, P, Q)
+
+
+def test_separate_sweeps_still_bit_exact(contexts):
+    ctx = contexts(RFD_FUSED=0)
+    ctx.reset_stats()
+    rng = np.random.default_rng(12)
+    for m, n in [(65, 66), (257, 100), (40, 700)]:
+        P, Q = walk(rng, m, 3), walk(rng, n, 3)
+        check_pair(ctx, P, Q)
+    st = ctx.stats()
+    assert "bounds_f64" not in st and st["dfd"]["launches"] == 3
+
+
+def test_fused_distance_matches_oracle_trajectory(contexts):
+    ctx = contexts()
+    P, Q = W.robot_pair(800, 700, seed=4)
+    ref = O.distance(P, Q, stop_mode=1, eps=1e-9)
+    with Intersection(P, Q, ctx=ctx) as s:
+        got = s.distance(stop="rel", eps=1e-9)
+    assert got["dfd"] == ref["dfd"] and got["vefd"] == ref["vefd"]
+    assert got["iterations"] == ref["iterations"] and got["distance"] == ref["distance"]

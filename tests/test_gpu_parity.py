@@ -133,10 +133,24 @@ def test_mixed_precision_result_within_tolerance(gpu):
     assert got["lb"] <= ref["ub"] and ref["lb"] <= got["ub"]
 
 
+@pytest.fixture(scope="module")
+def pruning_ctx():
+    """A context whose FP64 bounds run as separate sweeps with VE-FD pruning."""
+    os.environ["RFD_PRUNE"] = "1"
+    try:
+        ctx = Context()
+    finally:
+        del os.environ["RFD_PRUNE"]
+    yield ctx
+    ctx.close()
+
+
 @pytest.mark.parametrize("kind", ["walk2", "walk3", "robot"])
-def test_pruned_and_skipped_sweeps_bit_exact(gpu, kind):
+def test_pruned_and_skipped_sweeps_bit_exact(pruning_ctx, kind):
     """Sizes where VE-FD pruning (cells above DFD^2) and 64-step dead-chunk
     skipping both occur; results must not change."""
+    gpu = pruning_ctx
+    gpu.reset_stats()
     if kind == "robot":
         P, Q = W.robot_pair(1500, 1800, seed=3)
     else:
@@ -153,6 +167,6 @@ def test_pruned_and_skipped_sweeps_bit_exact(gpu, kind):
         _, gv32 = s.Bound_call_gpu(lb, fp32=True)
         assert gv32 == O.vefd(P, Q, lb, fp32=True)
         st = gpu.stats()
-        assert st["vefd_f64"]["evaluated"] <= st["vefd_f64"]["cells"]
+        assert st["vefd_f64"]["evaluated"] < st["vefd_f64"]["cells"]
         for r in (lb, 0.5 * (lb + gv), gv * (1 - 1e-9), gv, 0.5 * (gv + dfd), dfd):
             assert s.decide_ub(r) == O.decide_ub(P, Q, r)
