@@ -35,8 +35,8 @@ KERNEL_SYMBOL = {k: f"rfd::band_sweep<{op}, 3, {t}, 1, {_R}, false>" for k, op, 
                  (("vefd_f64", 1, "double"), ("vefd_f32", 1, "float"), ("decide_f64", 2, "double"),
                   ("decide_f32", 2, "float"), ("dfd", 0, "double"))}
 # Fused DFD + VE-FD sweep: bounds_sweep<dims, rows per lane, producer waves>;
-# at 200k rows the library picks 13 rows per lane and 6 producer waves.
-_BCFG = int(os.environ.get("RFD_BOUNDS", "136"))
+# at 200k rows the library picks 13 rows per lane and 7 producer waves.
+_BCFG = int(os.environ.get("RFD_BOUNDS", "137"))
 KERNEL_SYMBOL["bounds_f64"] = f"rfd::bounds_sweep<3, {_BCFG // 10}, {_BCFG % 10}>"
 
 

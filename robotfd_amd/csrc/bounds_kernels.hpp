@@ -129,21 +129,24 @@ __device__ inline double dmax(double a, double b) {
 // row's own values from the previous column:
 //   DFD   (.cu:458-641)  top = max(d, min(dl, dbot)),  dl' = min(top, dbot)
 //   VE-FD (.cu:200-432)  in = min(vl, vbot),  T = max(wt, in),  vl' = R = max(wr, in)
-template <int R>
-__device__ inline void chain_cells(const double *w, double (&dbot)[R], double (&vbot)[R], double &dl,
+// Rows G at a time (one LDS round trip per group): G = R up to 8 rows per
+// lane, half of R above, which bounds the registers the weights hold.
+template <int R, int T0, int G>
+__device__ inline void chain_group(const double *w, double (&dbot)[R], double (&vbot)[R], double &dl,
                                    double &vl) {
-    double wv[R][3];
+    double wv[G][3];
 #pragma unroll
-    for (int t = 0; t < R; ++t)
+    for (int g = 0; g < G; ++g)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) wv[t][c] = w[(t * 3 + c) * 64];
+        for (int c = 0; c < 3; ++c) wv[g][c] = w[((T0 + g) * 3 + c) * 64];
 #pragma unroll
-    for (int t = 0; t < R; ++t)
+    for (int g = 0; g < G; ++g)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) asm volatile("" ::"v"(wv[t][c]));
+        for (int c = 0; c < 3; ++c) asm volatile("" ::"v"(wv[g][c]));
 #pragma unroll
-    for (int t = 0; t < R; ++t) {
-        const double d = wv[t][0], wr = wv[t][1], wt = wv[t][2];
+    for (int g = 0; g < G; ++g) {
+        const int t = T0 + g;
+        const double d = wv[g][0], wr = wv[g][1], wt = wv[g][2];
         const double top = dmax(d, dmin(dl, dbot[t]));
         dl = dmin(top, dbot[t]);
         dbot[t] = top;
@@ -151,6 +154,14 @@ __device__ inline void chain_cells(const double *w, double (&dbot)[R], double (&
         vbot[t] = dmax(wt, vin);
         vl = dmax(wr, vin);
     }
+}
+
+template <int R>
+__device__ inline void chain_cells(const double *w, double (&dbot)[R], double (&vbot)[R], double &dl,
+                                   double &vl) {
+    constexpr int G = (R <= 8) ? R : (R + 1) / 2;
+    chain_group<R, 0, G>(w, dbot, vbot, dl, vl);
+    if constexpr (G < R) chain_group<R, G, R - G>(w, dbot, vbot, dl, vl);
 }
 
 // Columns of the band-boundary rings in LDS (incoming and outgoing).

@@ -240,7 +240,7 @@ hipError_t launch_bounds_cfg(rfd_ctx *ctx, int64_t nb, const rfd::BoundsArgs &a,
 
 bool valid_bounds_cfg(int cfg) {
     return cfg == 42 || cfg == 62 || cfg == 63 || cfg == 72 || cfg == 73 || cfg == 83 || cfg == 84 || cfg == 21 ||
-           cfg == 136;
+           cfg == 136 || cfg == 137;
 }
 
 template <int D>
@@ -254,6 +254,7 @@ hipError_t launch_bounds(rfd_ctx *ctx, int cfg, int64_t nb, const rfd::BoundsArg
         case 83: return launch_bounds_cfg<D, 8, 3>(ctx, nb, a, grid);
         case 84: return launch_bounds_cfg<D, 8, 4>(ctx, nb, a, grid);
         case 136: return launch_bounds_cfg<D, 13, 6>(ctx, nb, a, grid);
+        case 137: return launch_bounds_cfg<D, 13, 7>(ctx, nb, a, grid);
         default: return launch_bounds_cfg<D, 4, 2>(ctx, nb, a, grid);
     }
 }
@@ -262,12 +263,12 @@ hipError_t launch_bounds(rfd_ctx *ctx, int cfg, int64_t nb, const rfd::BoundsArg
 rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, double *vefd) {
     rfd_ctx *ctx = s->ctx;
     // Configuration: RFD_BOUNDS if set; otherwise 13 rows per lane (one band
-    // per CU, 6 producer waves) when that still gives most CUs a band, else 4
+    // per CU, 7 producer waves) when that still gives most CUs a band, else 4
     // rows per lane with 2 producer waves (several bands per CU).
     int cfg = ctx->bounds_cfg;
     if (cfg == 0) {
         const int64_t nb13 = (s->m + 64 * 13 - 1) / (64 * 13);
-        cfg = (4 * nb13 >= 3 * (int64_t)ctx->cu_count) ? 136 : 42;
+        cfg = (4 * nb13 >= 3 * (int64_t)ctx->cu_count) ? 137 : 42;
     }
     const int R = cfg / 10;
     const int64_t rows = s->m, cols = s->n;

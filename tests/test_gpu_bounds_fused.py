@@ -61,7 +61,7 @@ def check_pair(ctx, P, Q):
     assert ve == O.vefd(P, Q, lb)
 
 
-@pytest.mark.parametrize("cfg", [21, 42, 62, 63, 72, 73, 83, 84, 136])
+@pytest.mark.parametrize("cfg", [21, 42, 62, 63, 72, 73, 83, 84, 136, 137])
 @pytest.mark.parametrize("dims", [2, 3])
 def test_fused_configs_bit_exact(contexts, cfg, dims):
     ctx = contexts(RFD_BOUNDS=cfg)
@@ -75,7 +75,7 @@ def test_fused_configs_bit_exact(contexts, cfg, dims):
     assert st["bounds_f64"]["launches"] == len(SHAPES) and "dfd" not in st
 
 
-@pytest.mark.parametrize("cfg", [42, 84, 136])
+@pytest.mark.parametrize("cfg", [42, 84, 136, 137])
 def test_fused_workgroups_take_several_bands(contexts, cfg):
     ctx = contexts(RFD_BOUNDS=cfg, RFD_MAX_WAVES=3)
     rng = np.random.default_rng(5)
