@@ -13,6 +13,7 @@
 // is fma(0, 0, acc) = acc).
 #pragma once
 
+#include "bounds_kernels.hpp"
 #include "sweep_kernels.hpp"
 
 namespace rfd {
@@ -27,7 +28,8 @@ __device__ inline uint16_t f2bf_dev(float f) {
 
 namespace rfd {
 
-enum BatchPhase { PH_DFD = 0, PH_VEFD = 1, PH_VEFD_REDO = 2, PH_DEC = 3, PH_CERT = 4 };
+// PH_BOUNDS: DFD and VE-FD of every pair in one fused round (bounds_kernels.hpp).
+enum BatchPhase { PH_DFD = 0, PH_VEFD = 1, PH_VEFD_REDO = 2, PH_DEC = 3, PH_CERT = 4, PH_BOUNDS = 5 };
 
 // Speculative bisection.  A decision round may evaluate the next d levels of a
 // pair's bisection tree at once: node 0 splits (lb, ub); node i's children
@@ -79,6 +81,7 @@ struct BatchArgs {
     unsigned long long *evals;
     unsigned long long *t0;  // wall clock at batch start
     int rows_per_lane;
+    int bounds_rows;         // rows per lane of the fused bound round
     int stop_mode;
     double eps;
     int prune;
@@ -92,7 +95,7 @@ __device__ inline bool batch_keep(double lb, double ub, int stop_mode, double ep
 }
 
 __device__ inline long long batch_rows(const BatchPair &p, int phase) {
-    return phase == PH_DFD ? p.m : p.m - 1;
+    return (phase == PH_DFD || phase == PH_BOUNDS) ? p.m : p.m - 1;
 }
 
 __device__ inline bool batch_runs_vefd(const BatchPair &p, int gate) {
@@ -101,7 +104,7 @@ __device__ inline bool batch_runs_vefd(const BatchPair &p, int gate) {
 
 // Whether pair k takes part in a round of `phase`.
 __device__ inline bool batch_in_round(const BatchPair &p, const BatchState &st, int phase, int gate) {
-    if (phase == PH_DFD) return true;
+    if (phase == PH_DFD || phase == PH_BOUNDS) return true;
     if (p.m < 2 || p.n < 2) return false;
     if (phase == PH_VEFD) return batch_runs_vefd(p, gate);
     if (phase == PH_VEFD_REDO) return st.need_redo != 0;
@@ -166,7 +169,7 @@ __global__ __launch_bounds__(1024) void batch_plan(BatchArgs a, int phase) {
     const int tid = (int)threadIdx.x;
     const int per = (a.npairs + 1023) / 1024;
     const int k0 = tid * per;
-    const int R = a.rows_per_lane;
+    const int R = (phase == PH_BOUNDS) ? a.bounds_rows : a.rows_per_lane;
     long long mine = 0;
     for (int q = 0; q < per; ++q) {
         const int idx = k0 + q;
@@ -248,6 +251,10 @@ __global__ __launch_bounds__(1024) void batch_plan(BatchArgs a, int phase) {
             pos += nb;
             st.result[node][0] = -3.0;
             st.result[node][1] = -3.0;
+            if (phase == PH_BOUNDS) {  // the fused round writes nodes 0 and 1
+                st.result[1][0] = -3.0;
+                st.result[1][1] = -3.0;
+            }
         }
         if (phase == PH_VEFD || phase == PH_VEFD_REDO) {
             const double F = st.F;
@@ -297,6 +304,42 @@ __global__ __launch_bounds__(64) void batch_sweep(BatchArgs a) {
     if (a.evals) atomicAdd(a.evals, evals);
 }
 
+// Fused bound round: workgroups take (pair, band) items in order and run the
+// band with the single-pair fused code; a pair's DFD lands in its node-0
+// result, its VE-FD in node 1's, and the band hand-off uses nodes 0 and 1's
+// boundary columns (4n values).
+template <int R, int NPW>
+__global__ __launch_bounds__(64 * (NPW + 2)) void batch_bounds(BatchArgs a) {
+    __shared__ BoundsShared<R, NPW> sh;
+    const int total = __hip_atomic_load(&a.count[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        if (threadIdx.x == 0) {
+            sh.item = atomicAdd(&a.count[1], 1);
+            bounds_clear(sh);
+        }
+        __syncthreads();
+        const int idx = sh.item;
+        if (idx >= total) break;
+        const int2 it = a.items[idx];
+        const int k = it.x / kSpec;
+        const BatchPair p = a.pairs[k];
+        BoundsArgs b;
+        b.P = a.pts + p.poff;
+        b.Q = a.pts + p.qoff;
+        b.slots = a.slots + p.sloff;
+        b.progress = a.progress + p.bandoff;
+        b.queue = nullptr;
+        b.result = &a.state[k].result[0][0];
+        b.rows = p.m;
+        b.cols = p.n;
+        b.nbands = (int)((p.m + 64LL * R - 1) / (64LL * R));
+        b.r2 = p.endpoint_lb * p.endpoint_lb;
+        b.tlog = nullptr;
+        bounds_band<3, R, NPW>(b, it.y, sh);
+        __syncthreads();
+    }
+}
+
 // Decision per-column terms of every pair (4*Ab, 0.5/Ab), once per batch.
 __global__ void batch_qseg(BatchArgs a) {
     const int k = (int)blockIdx.y;
@@ -318,7 +361,25 @@ __global__ void batch_update(BatchArgs a, int phase) {
     const BatchPair &p = a.pairs[k];
     BatchState &st = a.state[k];
     if (!batch_in_round(p, st, phase, a.gate)) return;
-    if (phase == PH_DFD) {
+    if (phase == PH_BOUNDS) {
+        st.dfd = st.result[0][0];
+        st.F = st.result[0][1];
+        st.ub = st.dfd;
+        st.lb = p.endpoint_lb;
+        st.need_redo = 0;
+        if (p.m < 2 || p.n < 2) {
+            // One point against a curve: the DFD is the distance.
+            st.lb = st.ub;
+            st.distance = st.dfd;
+            st.active = 0;
+            st.t_done = wall_clock64();
+            return;
+        }
+        if (batch_runs_vefd(p, a.gate)) {
+            st.vefd = st.result[1][0];
+            st.lb = st.vefd;
+        }
+    } else if (phase == PH_DFD) {
         st.dfd = st.result[0][0];
         st.F = st.result[0][1];
         st.ub = st.dfd;
@@ -339,7 +400,9 @@ __global__ void batch_update(BatchArgs a, int phase) {
         }
         return;
     }
-    if (phase == PH_VEFD || phase == PH_VEFD_REDO) {
+    if (phase == PH_BOUNDS) {
+        // the bracket is set above; the common tail below applies the stop rule
+    } else if (phase == PH_VEFD || phase == PH_VEFD_REDO) {
         if (phase == PH_VEFD && st.result[0][1] >= st.thr) {  // pruning reached: rerun unpruned
             st.need_redo = 1;
             return;

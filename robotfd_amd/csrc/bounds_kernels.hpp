@@ -486,37 +486,55 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
     }
 }
 
+// LDS of one workgroup of the fused sweep.
+template <int R, int NPW> struct BoundsShared {
+    static constexpr int KS = BoundsRing<R>::KS;
+    double wring[KS * R * 3 * 64];
+    double inring[2 * kRing];
+    double outring[2 * kRing];
+    int sprod[NPW];
+    int swords[4];  // cons, in_avail, out_avail, out_done
+    int item;
+};
+
+// One band of one pair, by every wave of the workgroup.  The caller has
+// cleared the sync words and ends with a barrier before reusing the LDS.
+template <int D, int R, int NPW>
+__device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsShared<R, NPW> &sh) {
+    constexpr int KS = BoundsRing<R>::KS;
+    const int wave = (int)(threadIdx.x >> 6);
+    const BoundsSync sy{sh.sprod, &sh.swords[0], &sh.swords[1], &sh.swords[2], &sh.swords[3]};
+    // Waves of a workgroup are dealt to the CU's four SIMDs in turn, so wave 4
+    // shares a SIMD with wave 0: that is the courier, which mostly sleeps, and
+    // the chain's SIMD keeps its issue slots for the chain.
+    constexpr int cw = (NPW + 2 > 4) ? 4 : NPW + 1;
+    if (wave == 0)
+        bounds_chain<R, KS>(a, band, sh.wring, sh.inring, sh.outring, sy, NPW);
+    else if (wave == cw)
+        bounds_courier(a, band, sh.inring, sh.outring, sy);
+    else
+        bounds_producer<D, R, NPW, KS>(a, band, wave - 1 - (wave > cw ? 1 : 0), sh.wring, sh.sprod, sy.cons);
+}
+
+template <int R, int NPW>
+__device__ inline void bounds_clear(BoundsShared<R, NPW> &sh) {
+    for (int w = 0; w < NPW; ++w) sh.sprod[w] = 0;
+    for (int k = 0; k < 4; ++k) sh.swords[k] = 0;
+}
+
 template <int D, int R, int NPW>
 __global__ __launch_bounds__(64 * (NPW + 2)) void bounds_sweep(BoundsArgs a) {
     static_assert(NPW >= 1 && NPW <= R, "one to R producer waves");
-    constexpr int KS = BoundsRing<R>::KS;
-    __shared__ double wring[KS * R * 3 * 64];
-    __shared__ double inring[2 * kRing];
-    __shared__ double outring[2 * kRing];
-    __shared__ int sprod[NPW];
-    __shared__ int swords[4];  // cons, in_avail, out_avail, out_done
-    __shared__ int sband;
-    const int wave = (int)(threadIdx.x >> 6);
-    const BoundsSync sy{sprod, &swords[0], &swords[1], &swords[2], &swords[3]};
+    __shared__ BoundsShared<R, NPW> sh;
     for (;;) {
         if (threadIdx.x == 0) {
-            sband = atomicAdd(a.queue, 1);
-            for (int w = 0; w < NPW; ++w) sprod[w] = 0;
-            for (int k = 0; k < 4; ++k) swords[k] = 0;
+            sh.item = atomicAdd(a.queue, 1);
+            bounds_clear(sh);
         }
         __syncthreads();
-        const int band = sband;
+        const int band = sh.item;
         if (band >= a.nbands) break;
-        // Waves of a workgroup are dealt to the CU's four SIMDs in turn, so
-        // wave 4 shares a SIMD with wave 0: that is the courier, which mostly
-        // sleeps, and the chain's SIMD keeps its issue slots for the chain.
-        constexpr int cw = (NPW + 2 > 4) ? 4 : NPW + 1;
-        if (wave == 0)
-            bounds_chain<R, KS>(a, band, wring, inring, outring, sy, NPW);
-        else if (wave == cw)
-            bounds_courier(a, band, inring, outring, sy);
-        else
-            bounds_producer<D, R, NPW, KS>(a, band, wave - 1 - (wave > cw ? 1 : 0), wring, sprod, sy.cons);
+        bounds_band<D, R, NPW>(a, band, sh);
         __syncthreads();
     }
 }

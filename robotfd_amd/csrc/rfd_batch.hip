@@ -37,10 +37,11 @@ struct GraphKey {
     double eps;
     const void *net;   // learned runs: the network whose forward pass is captured
     double tau, delta;
+    int bcfg;          // fused bound round configuration (0: separate DFD and VE-FD rounds)
     bool operator<(const GraphKey &o) const {
-        return std::tie(rpl, stop_mode, prune, gate, rounds, with_bounds, depth, eps, net, tau, delta) <
+        return std::tie(rpl, stop_mode, prune, gate, rounds, with_bounds, depth, eps, net, tau, delta, bcfg) <
                std::tie(o.rpl, o.stop_mode, o.prune, o.gate, o.rounds, o.with_bounds, o.depth, o.eps,
-                        o.net, o.tau, o.delta);
+                        o.net, o.tau, o.delta, o.bcfg);
     }
 };
 
@@ -96,8 +97,29 @@ void launch_batch_sweep(int rpl, int grid, hipStream_t st, const rfd::BatchArgs 
     else launch_batch_sweep_p<OP, false>(rpl, grid, st, a);
 }
 
+template <int R, int NPW>
+void launch_batch_bounds_cfg(rfd_ctx *ctx, hipStream_t st, const rfd::BatchArgs &a) {
+    auto kern = rfd::batch_bounds<R, NPW>;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * (NPW + 2), 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    const int grid = std::max(1, std::min(ctx->cu_count * per_cu, ctx->max_waves));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * (NPW + 2)), 0, st, a);
+}
+
+
+void launch_batch_bounds(rfd_ctx *ctx, int cfg, hipStream_t st, const rfd::BatchArgs &a) {
+    switch (cfg) {
+        case 21: launch_batch_bounds_cfg<2, 1>(ctx, st, a); break;
+        case 63: launch_batch_bounds_cfg<6, 3>(ctx, st, a); break;
+        case 137: launch_batch_bounds_cfg<13, 7>(ctx, st, a); break;
+        default: launch_batch_bounds_cfg<4, 2>(ctx, st, a); break;
+    }
+}
+
 rfd::BatchArgs make_args(rfd_batch *b, int rpl, const rfd_opts &o, int prune) {
     rfd::BatchArgs a{};
+    a.bounds_rows = b->ctx->batch_bounds / 10;
     a.pairs = b->pairs;
     a.state = b->state;
     a.order = b->d_order;
@@ -136,12 +158,19 @@ void enqueue_rounds(rfd_batch *b, const rfd::BatchArgs &a, bool with_bounds, int
     if (with_bounds) {
         hipLaunchKernelGGL(rfd::batch_reset, dim3(pg), dim3(256), 0, st, a);
         hipLaunchKernelGGL(rfd::batch_qseg, dim3(8, (unsigned)b->npairs), dim3(256), 0, st, a);
-        const int phases[3] = {rfd::PH_DFD, rfd::PH_VEFD, rfd::PH_VEFD_REDO};
-        for (int ph : phases) {
-            hipLaunchKernelGGL(rfd::batch_plan, dim3(1), dim3(1024), 0, st, a, ph);
-            if (ph == rfd::PH_DFD) launch_batch_sweep<rfd::OP_DFD>(rpl, grid, st, a);
-            else launch_batch_sweep<rfd::OP_VEFD>(rpl, grid, st, a, ph == rfd::PH_VEFD && a.prune);
-            hipLaunchKernelGGL(rfd::batch_update, dim3(pg), dim3(256), 0, st, a, ph);
+        if (ctx->batch_bounds) {
+            // DFD and VE-FD of every pair in one fused round.
+            hipLaunchKernelGGL(rfd::batch_plan, dim3(1), dim3(1024), 0, st, a, (int)rfd::PH_BOUNDS);
+            launch_batch_bounds(ctx, ctx->batch_bounds, st, a);
+            hipLaunchKernelGGL(rfd::batch_update, dim3(pg), dim3(256), 0, st, a, (int)rfd::PH_BOUNDS);
+        } else {
+            const int phases[3] = {rfd::PH_DFD, rfd::PH_VEFD, rfd::PH_VEFD_REDO};
+            for (int ph : phases) {
+                hipLaunchKernelGGL(rfd::batch_plan, dim3(1), dim3(1024), 0, st, a, ph);
+                if (ph == rfd::PH_DFD) launch_batch_sweep<rfd::OP_DFD>(rpl, grid, st, a);
+                else launch_batch_sweep<rfd::OP_VEFD>(rpl, grid, st, a, ph == rfd::PH_VEFD && a.prune);
+                hipLaunchKernelGGL(rfd::batch_update, dim3(pg), dim3(256), 0, st, a, ph);
+            }
         }
         if (net) {
             hipLaunchKernelGGL(rfd::batch_features, dim3((unsigned)b->npairs), dim3(128), 0, st, a,
@@ -195,7 +224,7 @@ rfd_status run_once(rfd_batch *b, const rfd_opts &o, int rpl, int prune,
     if (o.use_graph) {
         hipGraphExec_t ge;
         GraphKey key{rpl, o.stop_mode, prune, o.vefd_gate, kDecisionRounds, 1, a.max_depth, o.eps,
-                     net, tau, delta};
+                     net, tau, delta, ctx->batch_bounds};
         rfd_status s = get_graph(b, key, a, &ge, net);
         if (s != RFD_OK) return s;
         RFD_HIP(ctx, hipGraphLaunch(ge, ctx->stream));
@@ -218,7 +247,7 @@ rfd_status run_once(rfd_batch *b, const rfd_opts &o, int rpl, int prune,
         if (o.use_graph) {
             hipGraphExec_t ge;
             GraphKey key{rpl, o.stop_mode, prune, o.vefd_gate, kContinueRounds, 0, a.max_depth, o.eps,
-                         nullptr, 0.0, 0.0};
+                         nullptr, 0.0, 0.0, ctx->batch_bounds};
             rfd_status s = get_graph(b, key, a, &ge);
             if (s != RFD_OK) return s;
             RFD_HIP(ctx, hipGraphLaunch(ge, ctx->stream));

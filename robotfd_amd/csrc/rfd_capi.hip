@@ -408,6 +408,10 @@ rfd_status rfd_create(rfd_ctx **out, int device) {
         if (valid_bounds_cfg(c)) ctx->bounds_cfg = c;
     }
     if (const char *e = std::getenv("RFD_WGCU")) ctx->wg_per_cu = std::max(0, std::atoi(e));
+    if (const char *e = std::getenv("RFD_BATCH_BOUNDS")) {
+        const int c = std::atoi(e);
+        if (c == 0 || c == 21 || c == 42 || c == 63 || c == 137) ctx->batch_bounds = c;
+    }
     {
         int lds = 0;
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) == hipSuccess)

@@ -1,13 +1,14 @@
 """Batched pipeline parity: every pair of a batch against the CPU oracle,
 bit-exact, for every tile height, with and without the hipGraph, and across
 graph replays."""
+import os
 import sys
 
 import numpy as np
 import pytest
 
 from oracle import oracle as O
-from robotfd_amd import workloads as W
+from robotfd_amd import Context, workloads as W
 from robotfd_amd.batch import Batch
 
 pytestmark = pytest.mark.gpu
@@ -67,12 +68,49 @@ def test_batch_graph_replay_and_stream_agree(batch_and_ref):
     assert (a["latency_ms"] > 0).all()
 
 
+def ctx_with(**env):
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update({k: str(v) for k, v in env.items()})
+    try:
+        return Context()
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("bcfg", [0, 21, 42, 63, 137])
+def test_batch_bound_round_configs(batch_and_ref, bcfg):
+    """Bounds as separate DFD and VE-FD rounds (0) or as one fused round in
+    each configuration (rows per lane, producer waves)."""
+    _, refs = batch_and_ref
+    ctx = ctx_with(RFD_BATCH_BOUNDS=bcfg)
+    try:
+        with Batch(mixed_pairs(40, seed=5), ctx=ctx) as b:
+            print("batch bounds cfg", bcfg, flush=True)
+            out = b.run(stop="rel", eps=1e-9, rows_per_lane=2, spec_depth=1)
+        for k, ref in enumerate(refs):
+            assert out["dfd"][k] == ref["dfd"], k
+            assert out["vefd"][k] == ref["vefd"], k
+            assert out["iterations"][k] == ref["iterations"], k
+            assert out["distance"][k] == ref["distance"], k
+    finally:
+        ctx.close()
+
+
 def test_batch_pruning_does_not_change_results(batch_and_ref):
-    b, refs = batch_and_ref
-    on = b.run(rows_per_lane=4, prune=1)
-    off = b.run(rows_per_lane=4, prune=0)
-    for key in ("distance", "dfd", "vefd", "iterations"):
-        assert np.array_equal(on[key], off[key]), key
+    """Pruning applies to the separate VE-FD round."""
+    ctx = ctx_with(RFD_BATCH_BOUNDS=0)
+    try:
+        with Batch(mixed_pairs(40, seed=5), ctx=ctx) as b:
+            on = b.run(rows_per_lane=4, prune=1)
+            off = b.run(rows_per_lane=4, prune=0)
+        for key in ("distance", "dfd", "vefd", "iterations"):
+            assert np.array_equal(on[key], off[key]), key
+    finally:
+        ctx.close()
 
 
 def test_batch_reference_stop_rule(batch_and_ref):
