@@ -106,6 +106,25 @@ def cpu_baseline(P, Q, mix: dict, dims: int, budget_s: float = 12.0) -> dict:
                       "; rates applied to the GPU run's per-sweep cell counts"}
 
 
+def precision_form(sess, ctx, ref: dict) -> dict:
+    """One distance computation of the same resident pair in mixed precision:
+    FP32 VE-FD weights and FP32 bisection while the bracket is wide, FP64 DFD,
+    FP64 certification of the FP32 bracket and FP64 final steps.  Reported
+    beside the FP64 line with its error against the FP64 result."""
+    ctx.reset_stats()
+    t0 = time.perf_counter()
+    r = sess.distance(stop="rel", eps=1e-9, precision="mixed")
+    wall = time.perf_counter() - t0
+    st = ctx.stats()
+    cells = sum(v["cells"] for v in st.values())
+    return {"distance": r["distance"], "fp64_distance": ref["distance"],
+            "rel_error_vs_fp64": abs(r["distance"] - ref["distance"]) / ref["distance"],
+            "err_bound": r["err_bound"], "iterations": r["iterations"],
+            "iterations_fp32": r["iterations_fp32"], "certify_steps": r["certify_steps"],
+            "ms": wall * 1e3, "cells_per_s": cells / wall,
+            "kernels": {k: dict(v, avg_ms=v["ms"] / v["launches"]) for k, v in st.items()}}
+
+
 def batch_section(count: int, ctx) -> dict:
     """Pairs/s of the batched pipeline on `count` pairs (lengths log-uniform in
     [500, 20000] per curve, dims 2/3 at random, N(0,1) random walks, seed 0;
@@ -171,6 +190,8 @@ def main() -> int:
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--points", type=int, default=200_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-precision-form", action="store_true",
+                    help="skip the mixed-precision run reported beside the FP64 line")
     ap.add_argument("--batch-pairs", type=int, default=4096,
                     help="pairs of the batched workload reported beside the main line (0: skip)")
     args = ap.parse_args()
@@ -234,6 +255,8 @@ def main() -> int:
         "kernels": {k: dict(v, avg_ms=v["ms"] / v["launches"],
                             cells_per_s=v["cells"] / (v["ms"] * 1e-3)) for k, v in stats.items()},
     }
+    if args.precision == "fp64" and not args.no_precision_form:
+        out["precision_form"] = precision_form(sess, ctx, results[-1])
     sess.free_memory()
     if args.batch_pairs:
         out["batch"] = batch_section(args.batch_pairs, ctx)
