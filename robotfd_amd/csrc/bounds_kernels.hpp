@@ -300,15 +300,17 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, const d
 // The band's global-memory hand-offs, kept off the chain: the courier copies
 // the predecessor's published boundary columns into the incoming ring and
 // publishes the chain's outgoing columns (scoped stores, s_waitcnt, flag), as
-// many columns per round as are ready, up to 64.
+// many columns per round as are ready, up to 64.  NK values per column
+// (bound sweep: DFD and VE-FD; decision: one); slots are [buffer][NK][cols].
+template <int NK = 2>
 __device__ inline void bounds_courier(const BoundsArgs &a, const int band, double *inring,
                                       const double *outring, const BoundsSync &sy) {
     const int lane = (int)(threadIdx.x & 63);
     const long long cols = a.cols;
-    const double *inD = a.slots + (long long)((band + 1) & 1) * 2 * cols;
-    const double *inV = inD + cols;
-    double *outD = a.slots + (long long)(band & 1) * 2 * cols;
-    double *outV = outD + cols;
+    const double *inD = a.slots + (long long)((band + 1) & 1) * NK * cols;
+    const double *inV = inD + (NK - 1) * cols;
+    double *outD = a.slots + (long long)(band & 1) * NK * cols;
+    double *outV = outD + (NK - 1) * cols;
     long long in_next = band > 0 ? 0 : cols;
     long long out_next = band + 1 < a.nbands ? 0 : cols;
     while (in_next < cols || out_next < cols) {
@@ -326,9 +328,11 @@ __device__ inline void bounds_courier(const BoundsArgs &a, const int band, doubl
                     if (lane < n) {
                         const long long c = in_next + lane;
                         const double vD = column_load<SYNC_SCOPED>(inD + c);
-                        const double vV = column_load<SYNC_SCOPED>(inV + c);
                         inring[c % kRing] = vD;
-                        inring[kRing + c % kRing] = vV;
+                        if (NK > 1) {
+                            const double vV = column_load<SYNC_SCOPED>(inV + c);
+                            inring[kRing + c % kRing] = vV;
+                        }
                     }
                     __atomic_signal_fence(__ATOMIC_SEQ_CST);
                     __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the ring writes are done
@@ -348,7 +352,7 @@ __device__ inline void bounds_courier(const BoundsArgs &a, const int band, doubl
                 if (lane < n) {
                     const long long c = out_next + lane;
                     column_store<SYNC_SCOPED>(outD + c, outring[c % kRing]);
-                    column_store<SYNC_SCOPED>(outV + c, outring[kRing + c % kRing]);
+                    if (NK > 1) column_store<SYNC_SCOPED>(outV + c, outring[kRing + c % kRing]);
                 }
                 __builtin_amdgcn_wave_barrier();
                 if (lane == 0) flag_store<SYNC_SCOPED>(&a.progress[band], (int)(out_next + n));
@@ -511,7 +515,7 @@ __device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsSh
     if (wave == 0)
         bounds_chain<R, KS>(a, band, sh.wring, sh.inring, sh.outring, sy, NPW);
     else if (wave == cw)
-        bounds_courier(a, band, sh.inring, sh.outring, sy);
+        bounds_courier<2>(a, band, sh.inring, sh.outring, sy);
     else
         bounds_producer<D, R, NPW, KS>(a, band, wave - 1 - (wave > cw ? 1 : 0), sh.wring, sh.sprod, sy.cons);
 }

@@ -8,6 +8,7 @@
 #include "rfd_internal.hpp"
 #include "sweep_kernels.hpp"
 #include "bounds_kernels.hpp"
+#include "dec_kernels.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -332,6 +333,76 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
     return RFD_OK;
 }
 
+// Fused decision sweep (dec_kernels.hpp): configurations as the bound sweep.
+template <int D, int R, int NPW>
+hipError_t launch_dec_cfg(rfd_ctx *ctx, int64_t nb, const rfd::BoundsArgs &a) {
+    auto kern = rfd::dec_sweep<D, R, NPW>;
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * (NPW + 2), 0);
+    if (e != hipSuccess) return e;
+    int64_t grid = std::min<int64_t>(nb, (int64_t)ctx->cu_count * std::max(per_cu, 1));
+    grid = std::max<int64_t>(1, std::min<int64_t>(grid, ctx->max_waves));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * (NPW + 2)), 0, ctx->stream, a);
+    return hipGetLastError();
+}
+
+bool valid_dec_cfg(int cfg) { return cfg == 21 || cfg == 42 || cfg == 63 || cfg == 137; }
+
+template <int D>
+hipError_t launch_dec(rfd_ctx *ctx, int cfg, int64_t nb, const rfd::BoundsArgs &a) {
+    switch (cfg) {
+        case 21: return launch_dec_cfg<D, 2, 1>(ctx, nb, a);
+        case 63: return launch_dec_cfg<D, 6, 3>(ctx, nb, a);
+        case 137: return launch_dec_cfg<D, 13, 7>(ctx, nb, a);
+        default: return launch_dec_cfg<D, 4, 2>(ctx, nb, a);
+    }
+}
+
+// One fused decision sweep at `radius`: the terminal rightUB (.cu:1592).
+rfd_status run_dec(rfd_session *s, double radius, double *result) {
+    rfd_ctx *ctx = s->ctx;
+    const int64_t rows = s->m - 1, cols = s->n - 1;
+    const int cfg = ctx->dec_cfg;
+    const int R = cfg / 10;
+    const int64_t nb = (rows + 64 * R - 1) / (64 * R);
+    rfd::BoundsArgs a{};
+    a.P = s->dP;
+    a.Q = s->dQ;
+    a.slots = (double *)s->slots;
+    a.progress = s->progress;
+    a.queue = s->queue;
+    a.result = s->result;
+    a.rows = rows;
+    a.cols = cols;
+    a.nbands = (int)nb;
+    a.r2 = radius * radius;
+    a.tlog = nullptr;
+    const double sentinel = -3.0;
+    RFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    hipLaunchKernelGGL(rfd::sweep_init, dim3((unsigned)std::min<int64_t>((nb + 255) / 256, 1024)),
+                       dim3(256), 0, ctx->stream, s->progress, (int)nb, s->queue, s->result, 1,
+                       sentinel, s->evals);
+    RFD_HIP(ctx, hipGetLastError());
+    RFD_HIP(ctx, s->dims == 2 ? launch_dec<2>(ctx, cfg, nb, a) : launch_dec<3>(ctx, cfg, nb, a));
+    RFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    double h = 0;
+    RFD_HIP(ctx, hipMemcpyAsync(&h, s->result, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    RFD_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    const double cells = (double)rows * (double)cols;
+    ctx->stats.launches[RFD_K_DEC64] += 1;
+    ctx->stats.ms[RFD_K_DEC64] += ms;
+    ctx->stats.cells[RFD_K_DEC64] += cells;
+    ctx->stats.evaluated[RFD_K_DEC64] += cells;
+    if (h == sentinel) {
+        set_err(ctx, "fused decision sweep wrote no terminal value");
+        return RFD_ERR_STATE;
+    }
+    *result = h;
+    return RFD_OK;
+}
+
 bool keep_bisecting(double lb, double ub, const rfd_opts &o, double eps) {
     return o.stop_mode == RFD_STOP_ABS ? (ub - lb > eps) : (ub - lb > eps * ub);
 }
@@ -408,6 +479,10 @@ rfd_status rfd_create(rfd_ctx **out, int device) {
         if (valid_bounds_cfg(c)) ctx->bounds_cfg = c;
     }
     if (const char *e = std::getenv("RFD_WGCU")) ctx->wg_per_cu = std::max(0, std::atoi(e));
+    if (const char *e = std::getenv("RFD_DEC")) {
+        const int c = std::atoi(e);
+        if (valid_dec_cfg(c)) ctx->dec_cfg = c;
+    }
     if (const char *e = std::getenv("RFD_BATCH_BOUNDS")) {
         const int c = std::atoi(e);
         if (c == 0 || c == 21 || c == 42 || c == 63 || c == 137) ctx->batch_bounds = c;
@@ -569,6 +644,11 @@ rfd_status rfd_session_decide(rfd_session *s, double radius, int fp32, int *reac
         if ((st = ensure_f32(s)) != RFD_OK) return st;
         if ((st = ensure_qseg<float>(s, &s->qseg32, s->fQ)) != RFD_OK) return st;
         st = run_sweep<rfd::OP_DEC, float>(s, s->fP, s->fQ, s->qseg32, radius, RFD_K_DEC32, &ub);
+    } else if (ctx->fused && ctx->dec_cfg != 0) {
+        // Opt-in (RFD_DEC): at 200k x 200k near the critical radius it takes
+        // 410 ms against 217 ms for band_sweep, which skips the free
+        // intervals of unreachable cells (about 80% of them there).
+        st = run_dec(s, radius, &ub);
     } else {
         if ((st = ensure_qseg<double>(s, &s->qseg64, s->dQ)) != RFD_OK) return st;
         st = run_sweep<rfd::OP_DEC, double>(s, s->dP, s->dQ, s->qseg64, radius, RFD_K_DEC64, &ub);

@@ -125,3 +125,49 @@ def test_fused_distance_matches_oracle_trajectory(contexts):
         got = s.distance(stop="rel", eps=1e-9)
     assert got["dfd"] == ref["dfd"] and got["vefd"] == ref["vefd"]
     assert got["iterations"] == ref["iterations"] and got["distance"] == ref["distance"]
+
+
+DEC_SHAPES = [(2, 2), (2, 3), (3, 2), (5, 7), (65, 66), (130, 200), (300, 257), (700, 50), (40, 700),
+              (1100, 90)]
+
+
+@pytest.mark.parametrize("cfg", [21, 42, 63, 137])
+@pytest.mark.parametrize("dims", [2, 3])
+def test_fused_decision_configs_bit_exact(contexts, cfg, dims):
+    """The fused decision sweep's terminal rightUB against the oracle's, at
+    radii from the endpoint bound to past DFD."""
+    ctx = contexts(RFD_DEC=cfg)
+    ctx.reset_stats()
+    calls = 0
+    for m, n in DEC_SHAPES:
+        rng = np.random.default_rng(7 * m + n + dims + cfg)
+        P, Q = walk(rng, m, dims), walk(rng, n, dims)
+        lb, ub = O.endpoint_lb(P, Q), O.dfd(P, Q)
+        say("fused decision", cfg, dims, (m, n))
+        with Intersection(P, Q, ctx=ctx) as s:
+            for r in np.linspace(lb, ub * 1.05 + 1e-12, 7):
+                assert s.decide_ub(r) == O.decide_ub(P, Q, r), (m, n, r)
+                calls += 1
+    assert ctx.stats()["decide_f64"]["launches"] == calls
+
+
+@pytest.mark.parametrize("cfg", [42, 137])
+def test_fused_decision_near_critical_value_and_wrap(contexts, cfg):
+    ctx = contexts(RFD_DEC=cfg, RFD_MAX_WAVES=3)
+    P, Q = W.robot_pair(64 * 13 * 4 + 7, 700, seed=2)
+    res = O.distance(P, Q, stop_mode=1, eps=1e-12)
+    fd = res["distance"]
+    with Intersection(P, Q, ctx=ctx) as s:
+        for f in (1 - 1e-6, 1 - 1e-9, 1 + 1e-9, 1 + 1e-6):
+            assert s.decide_ub(fd * f) == O.decide_ub(P, Q, fd * f), f
+
+
+def test_default_decision_sweep_still_bit_exact(contexts):
+    """Without RFD_DEC the decision runs on band_sweep (unreachable cells skipped)."""
+    ctx = contexts(RFD_FUSED=0)
+    rng = np.random.default_rng(21)
+    P, Q = walk(rng, 300, 3), walk(rng, 257, 3)
+    lb, ub = O.endpoint_lb(P, Q), O.dfd(P, Q)
+    with Intersection(P, Q, ctx=ctx) as s:
+        for r in np.linspace(lb, ub, 5):
+            assert s.decide_ub(r) == O.decide_ub(P, Q, r)
