@@ -125,6 +125,27 @@ def precision_form(sess, ctx, ref: dict) -> dict:
             "kernels": {k: dict(v, avg_ms=v["ms"] / v["launches"]) for k, v in st.items()}}
 
 
+def robot_section(ctx) -> dict:
+    """The long noisy-copy pair (3D robot-like paths of 100,000 and 120,000
+    points, seed 0): its bracket stays open, so the run is the fused bound
+    sweep plus about thirty FP64 decision sweeps of FP64 bisection to 1e-9
+    relative."""
+    from robotfd_amd import Intersection, workloads as W
+    P, Q = W.robot_pair(100_000, 120_000, seed=0)
+    with Intersection(P, Q, ctx=ctx) as s:
+        s.data_move()
+        ctx.reset_stats()
+        t0 = time.perf_counter()
+        r = s.distance(stop="rel", eps=1e-9)
+        wall = time.perf_counter() - t0
+    st = ctx.stats()
+    cells = sum(v["cells"] for v in st.values())
+    return {"workload": "3D robot-like pair 100,000 x 120,000 (noisy copy), seed 0; FP64 to 1e-9 relative",
+            "distance": r["distance"], "dfd": r["dfd"], "vefd": r["vefd"], "iterations": r["iterations"],
+            "ms": wall * 1e3, "cells_per_s": cells / wall,
+            "kernels": {k: dict(v, avg_ms=v["ms"] / v["launches"]) for k, v in st.items()}}
+
+
 def batch_section(count: int, ctx) -> dict:
     """Pairs/s of the batched pipeline on `count` pairs (lengths log-uniform in
     [500, 20000] per curve, dims 2/3 at random, N(0,1) random walks, seed 0;
@@ -190,6 +211,8 @@ def main() -> int:
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--points", type=int, default=200_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-robot", action="store_true",
+                    help="skip the 100k x 120k noisy-copy pair reported beside the main line")
     ap.add_argument("--no-precision-form", action="store_true",
                     help="skip the mixed-precision run reported beside the FP64 line")
     ap.add_argument("--batch-pairs", type=int, default=4096,
@@ -258,6 +281,8 @@ def main() -> int:
     if args.precision == "fp64" and not args.no_precision_form:
         out["precision_form"] = precision_form(sess, ctx, results[-1])
     sess.free_memory()
+    if not args.no_robot:
+        out["robot_pair"] = robot_section(ctx)
     if args.batch_pairs:
         out["batch"] = batch_section(args.batch_pairs, ctx)
     if not args.no_cpu_baseline:

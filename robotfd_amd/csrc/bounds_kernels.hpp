@@ -63,11 +63,11 @@ struct BoundsArgs {
 __device__ inline int lds_get(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 __device__ inline void lds_put(int *p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 
-// Ring depth in steps, 1536 R bytes each: about 48 KB per workgroup for up to
-// 8 rows per lane (several workgroups per CU), about 120 KB above that (one
-// workgroup per CU).
-template <int R> struct BoundsRing {
-    static constexpr int budget = (R <= 8) ? 49152 : 122880;
+// Ring depth in steps, 1536 R bytes each: about 48 KB per workgroup for the
+// configurations with few producers (several workgroups per CU), about 120 KB
+// for those with five or more (one workgroup per CU).
+template <int R, int NPW> struct BoundsRing {
+    static constexpr int budget = (NPW < 5) ? 49152 : 122880;
     static constexpr int KS = (budget / (1536 * R) < 16) ? budget / (1536 * R) : 16;
     static_assert(KS >= 2, "ring needs two slots");
 };
@@ -492,7 +492,7 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
 
 // LDS of one workgroup of the fused sweep.
 template <int R, int NPW> struct BoundsShared {
-    static constexpr int KS = BoundsRing<R>::KS;
+    static constexpr int KS = BoundsRing<R, NPW>::KS;
     double wring[KS * R * 3 * 64];
     double inring[2 * kRing];
     double outring[2 * kRing];
@@ -505,7 +505,7 @@ template <int R, int NPW> struct BoundsShared {
 // cleared the sync words and ends with a barrier before reusing the LDS.
 template <int D, int R, int NPW>
 __device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsShared<R, NPW> &sh) {
-    constexpr int KS = BoundsRing<R>::KS;
+    constexpr int KS = BoundsRing<R, NPW>::KS;
     const int wave = (int)(threadIdx.x >> 6);
     const BoundsSync sy{sh.sprod, &sh.swords[0], &sh.swords[1], &sh.swords[2], &sh.swords[3]};
     // Waves of a workgroup are dealt to the CU's four SIMDs in turn, so wave 4

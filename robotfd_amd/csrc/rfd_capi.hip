@@ -1,9 +1,10 @@
 // C ABI of the gfx950 continuous Frechet distance library (include/rfd.h).
 //
-// Host side of the path: sessions hold both curves resident in HBM, the
-// bound sweeps (DFD, VE-FD) and the decision sweep run as band-wavefront
-// kernels (sweep_kernels.hpp), and the bisection of frechet.cpp:18-75 runs on
-// the host between decision sweeps.
+// Host side of the path: sessions hold both curves resident in HBM; the FP64
+// bound sweeps (DFD, VE-FD) run as one fused sweep (bounds_kernels.hpp), the
+// decision sweep and the FP32 sweeps as band-wavefront kernels
+// (sweep_kernels.hpp); the bisection of frechet.cpp:18-75 runs on the host
+// between decision sweeps.
 #include "rfd.h"
 #include "rfd_internal.hpp"
 #include "sweep_kernels.hpp"
@@ -241,7 +242,7 @@ hipError_t launch_bounds_cfg(rfd_ctx *ctx, int64_t nb, const rfd::BoundsArgs &a,
 
 bool valid_bounds_cfg(int cfg) {
     return cfg == 42 || cfg == 62 || cfg == 63 || cfg == 72 || cfg == 73 || cfg == 83 || cfg == 84 || cfg == 21 ||
-           cfg == 136 || cfg == 137;
+           cfg == 136 || cfg == 137 || cfg == 77 || cfg == 107;
 }
 
 template <int D>
@@ -256,6 +257,8 @@ hipError_t launch_bounds(rfd_ctx *ctx, int cfg, int64_t nb, const rfd::BoundsArg
         case 84: return launch_bounds_cfg<D, 8, 4>(ctx, nb, a, grid);
         case 136: return launch_bounds_cfg<D, 13, 6>(ctx, nb, a, grid);
         case 137: return launch_bounds_cfg<D, 13, 7>(ctx, nb, a, grid);
+        case 77: return launch_bounds_cfg<D, 7, 7>(ctx, nb, a, grid);
+        case 107: return launch_bounds_cfg<D, 10, 7>(ctx, nb, a, grid);
         default: return launch_bounds_cfg<D, 4, 2>(ctx, nb, a, grid);
     }
 }
@@ -263,13 +266,15 @@ hipError_t launch_bounds(rfd_ctx *ctx, int cfg, int64_t nb, const rfd::BoundsArg
 // One fused bound sweep of a session (m, n >= 2): DFD, DFD^2 and VE-FD.
 rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, double *vefd) {
     rfd_ctx *ctx = s->ctx;
-    // Configuration: RFD_BOUNDS if set; otherwise 13 rows per lane (one band
-    // per CU, 7 producer waves) when that still gives most CUs a band, else 4
-    // rows per lane with 2 producer waves (several bands per CU).
+    // Configuration: RFD_BOUNDS if set; otherwise one band per CU with 7
+    // producer waves and the fewest rows per lane (7, 10 or 13) that cover
+    // the pair, or 4 rows per lane with 2 producer waves (several bands per
+    // CU) for pairs too short to give every CU a band of 7 rows per lane.
     int cfg = ctx->bounds_cfg;
     if (cfg == 0) {
-        const int64_t nb13 = (s->m + 64 * 13 - 1) / (64 * 13);
-        cfg = (4 * nb13 >= 3 * (int64_t)ctx->cu_count) ? 137 : 42;
+        const int64_t per = 64 * (int64_t)ctx->cu_count;
+        const int64_t need = (s->m + per - 1) / per;  // rows per lane for one band per CU
+        cfg = need <= 4 ? 42 : need <= 7 ? 77 : need <= 10 ? 107 : 137;
     }
     const int R = cfg / 10;
     const int64_t rows = s->m, cols = s->n;
@@ -705,6 +710,25 @@ rfd_status rfd_session_distance(rfd_session *s, const rfd_opts *opts, rfd_result
             ++iter32;
             if (r) ub = split; else lb = split;
         }
+        // The FP32 lower bound rarely meets DFD even where the FP64 one would
+        // (the FP64 bracket then closes at once).  One FP64 decision just
+        // below DFD settles the distance when it lies at DFD, the common end
+        // of open brackets; otherwise it is a certified upper end.
+        bool hi_certified = false;
+        if (keep_bisecting(lb, ub, o, eps) && ub == res->dfd) {
+            const double r = (o.stop_mode == RFD_STOP_ABS) ? ub - 0.5 * eps : ub * (1.0 - 0.5 * eps);
+            if (r > lb && r > s->endpoint_lb) {
+                int reach = 0;
+                if ((st = rfd_session_decide(s, r, 0, &reach, nullptr)) != RFD_OK) return st;
+                ++cert;
+                if (reach) {
+                    ub = r;
+                    hi_certified = true;
+                } else {
+                    lb = r;  // the distance lies in (r, DFD]
+                }
+            }
+        }
         // Certify the FP32 bracket with FP64 decisions at its ends: lb must be
         // unreachable and ub reachable.  Widen on failure.
         if (keep_bisecting(lb, ub, o, eps)) {
@@ -721,7 +745,7 @@ rfd_status rfd_session_distance(rfd_session *s, const rfd_opts *opts, rfd_result
             }
             for (int guard = 0; guard < 64; ++guard) {
                 int r = 0;
-                if (hi < res->dfd) {
+                if (hi < res->dfd && !(hi_certified && hi == ub)) {
                     if ((st = rfd_session_decide(s, hi, 0, &r, nullptr)) != RFD_OK) return st;
                     ++cert;
                     if (!r) { lo = hi; hi = std::min(res->dfd, hi + w); w *= 2; continue; }
