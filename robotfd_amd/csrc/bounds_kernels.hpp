@@ -91,13 +91,22 @@ __device__ inline double dpp_f64(double old, double src, int ctrl_shl) {
 
 // Squared distance of x to segment [a, b] given ab = b - a and |ab|^2; the
 // operation order of point_seg2 (.cu:375-394), with the segment terms hoisted.
+// The clamped parameter t = min(1, max(0, apab / ab2)) is 0 when apab <= 0
+// (or NaN) and 1 when apab >= ab2, whatever the quotient rounds to, so the
+// correctly rounded divide runs only for projections strictly inside the
+// segment: about 0.2% of point-segment pairs on long random walks, so most
+// waves skip it.
 template <int D>
 __device__ inline double seg_w(const double *x, const double *a, const double *ab, double ab2) {
     double apab = 0.0;
 #pragma unroll
     for (int k = 0; k < D; ++k) apab = __builtin_fma(x[k] - a[k], ab[k], apab);
-    double t = apab / ab2;
-    t = __builtin_fmin(1.0, __builtin_fmax(0.0, t));
+    double t = (apab > 0.0) ? 1.0 : 0.0;
+    if (apab > 0.0 && apab < ab2) {
+        asm volatile("");  // keep the divide in a branch, not a select
+        t = apab / ab2;
+        t = __builtin_fmin(1.0, __builtin_fmax(0.0, t));
+    }
     double w = 0.0;
 #pragma unroll
     for (int k = 0; k < D; ++k) {

@@ -110,8 +110,15 @@ __device__ inline T point_seg2(const T *x, const T *a, const T *b) {
         ab2 = N::fma(ab[k], ab[k], ab2);
         apab = N::fma(ap, ab[k], apab);
     }
-    T t = apab / ab2;
-    t = N::min(T(1), N::max(T(0), t));
+    // t = min(1, max(0, apab / ab2)) is 0 for apab <= 0 (or NaN) and 1 for
+    // apab >= ab2 whatever the quotient rounds to; the divide runs only for
+    // projections strictly inside the segment (a branch: most waves skip it).
+    T t = (apab > T(0)) ? T(1) : T(0);
+    if (apab > T(0) && apab < ab2) {
+        asm volatile("");
+        t = apab / ab2;
+        t = N::min(T(1), N::max(T(0), t));
+    }
     T w = T(0);
 #pragma unroll
     for (int k = 0; k < D; ++k) {
