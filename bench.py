@@ -35,9 +35,10 @@ KERNEL_SYMBOL = {k: f"rfd::band_sweep<{op}, 3, {t}, 1, {_R}, false>" for k, op, 
                  (("vefd_f64", 1, "double"), ("vefd_f32", 1, "float"), ("decide_f64", 2, "double"),
                   ("decide_f32", 2, "float"), ("dfd", 0, "double"))}
 # Fused DFD + VE-FD sweep: bounds_sweep<dims, rows per lane, producer waves>;
-# at 200k rows the library picks 13 rows per lane and 7 producer waves.
-_BCFG = int(os.environ.get("RFD_BOUNDS", "137"))
-KERNEL_SYMBOL["bounds_f64"] = f"rfd::bounds_sweep<3, {_BCFG // 10}, {_BCFG % 10}>"
+# at 200k rows the library picks 13 rows per lane and 10 producer waves.
+_BCFG = int(os.environ.get("RFD_BOUNDS", "1310"))
+_BR, _BN = (_BCFG // 100, _BCFG % 100) if _BCFG >= 1000 else (_BCFG // 10, _BCFG % 10)
+KERNEL_SYMBOL["bounds_f64"] = f"rfd::bounds_sweep<3, {_BR}, {_BN}>"
 
 
 def flops_per_launch(kind: str, dims: int, m: int, n: int, evaluated_per_launch: float) -> float:

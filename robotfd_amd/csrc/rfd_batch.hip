@@ -113,13 +113,15 @@ void launch_batch_bounds(rfd_ctx *ctx, int cfg, hipStream_t st, const rfd::Batch
         case 21: launch_batch_bounds_cfg<2, 1>(ctx, st, a); break;
         case 63: launch_batch_bounds_cfg<6, 3>(ctx, st, a); break;
         case 137: launch_batch_bounds_cfg<13, 7>(ctx, st, a); break;
+        case 77: launch_batch_bounds_cfg<7, 7>(ctx, st, a); break;
+        case 1310: launch_batch_bounds_cfg<13, 10>(ctx, st, a); break;
         default: launch_batch_bounds_cfg<4, 2>(ctx, st, a); break;
     }
 }
 
 rfd::BatchArgs make_args(rfd_batch *b, int rpl, const rfd_opts &o, int prune) {
     rfd::BatchArgs a{};
-    a.bounds_rows = b->ctx->batch_bounds / 10;
+    a.bounds_rows = b->ctx->batch_bounds >= 1000 ? b->ctx->batch_bounds / 100 : b->ctx->batch_bounds / 10;
     a.pairs = b->pairs;
     a.state = b->state;
     a.order = b->d_order;

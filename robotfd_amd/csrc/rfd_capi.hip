@@ -242,7 +242,7 @@ hipError_t launch_bounds_cfg(rfd_ctx *ctx, int64_t nb, const rfd::BoundsArgs &a,
 
 bool valid_bounds_cfg(int cfg) {
     return cfg == 42 || cfg == 62 || cfg == 63 || cfg == 72 || cfg == 73 || cfg == 83 || cfg == 84 || cfg == 21 ||
-           cfg == 136 || cfg == 137 || cfg == 77 || cfg == 107;
+           cfg == 136 || cfg == 137 || cfg == 77 || cfg == 107 || cfg == 1310 || cfg == 1313 || cfg == 1012;
 }
 
 template <int D>
@@ -259,6 +259,9 @@ hipError_t launch_bounds(rfd_ctx *ctx, int cfg, int64_t nb, const rfd::BoundsArg
         case 137: return launch_bounds_cfg<D, 13, 7>(ctx, nb, a, grid);
         case 77: return launch_bounds_cfg<D, 7, 7>(ctx, nb, a, grid);
         case 107: return launch_bounds_cfg<D, 10, 7>(ctx, nb, a, grid);
+        case 1310: return launch_bounds_cfg<D, 13, 10>(ctx, nb, a, grid);
+        case 1313: return launch_bounds_cfg<D, 13, 13>(ctx, nb, a, grid);
+        case 1012: return launch_bounds_cfg<D, 10, 10>(ctx, nb, a, grid);
         default: return launch_bounds_cfg<D, 4, 2>(ctx, nb, a, grid);
     }
 }
@@ -266,17 +269,18 @@ hipError_t launch_bounds(rfd_ctx *ctx, int cfg, int64_t nb, const rfd::BoundsArg
 // One fused bound sweep of a session (m, n >= 2): DFD, DFD^2 and VE-FD.
 rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, double *vefd) {
     rfd_ctx *ctx = s->ctx;
-    // Configuration: RFD_BOUNDS if set; otherwise one band per CU with 7
-    // producer waves and the fewest rows per lane (7, 10 or 13) that cover
-    // the pair, or 4 rows per lane with 2 producer waves (several bands per
-    // CU) for pairs too short to give every CU a band of 7 rows per lane.
+    // Configuration: RFD_BOUNDS if set; otherwise one band per CU with the
+    // fewest rows per lane (7, 10 or 13) that cover the pair (7 producer
+    // waves, 10 at 13 rows per lane), or 4 rows per lane with 2 producer waves
+    // (several bands per CU) for pairs too short to give every CU a band of 7
+    // rows per lane.
     int cfg = ctx->bounds_cfg;
     if (cfg == 0) {
         const int64_t per = 64 * (int64_t)ctx->cu_count;
         const int64_t need = (s->m + per - 1) / per;  // rows per lane for one band per CU
-        cfg = need <= 4 ? 42 : need <= 7 ? 77 : need <= 10 ? 107 : 137;
+        cfg = need <= 4 ? 42 : need <= 7 ? 77 : need <= 10 ? 107 : 1310;
     }
-    const int R = cfg / 10;
+    const int R = cfg >= 1000 ? cfg / 100 : cfg / 10;  // 10 R + NPW, or 100 R + NPW for NPW >= 10
     const int64_t rows = s->m, cols = s->n;
     const int64_t nb = (rows + 64 * R - 1) / (64 * R);
     rfd::BoundsArgs a{};
@@ -490,7 +494,7 @@ rfd_status rfd_create(rfd_ctx **out, int device) {
     }
     if (const char *e = std::getenv("RFD_BATCH_BOUNDS")) {
         const int c = std::atoi(e);
-        if (c == 0 || c == 21 || c == 42 || c == 63 || c == 137) ctx->batch_bounds = c;
+        if (c == 0 || c == 21 || c == 42 || c == 63 || c == 77 || c == 137 || c == 1310) ctx->batch_bounds = c;
     }
     {
         int lds = 0;

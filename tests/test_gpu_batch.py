@@ -81,7 +81,7 @@ def ctx_with(**env):
                 os.environ[k] = v
 
 
-@pytest.mark.parametrize("bcfg", [0, 21, 42, 63, 137])
+@pytest.mark.parametrize("bcfg", [0, 21, 42, 63, 77, 137, 1310])
 def test_batch_bound_round_configs(batch_and_ref, bcfg):
     """Bounds as separate DFD and VE-FD rounds (0) or as one fused round in
     each configuration (rows per lane, producer waves)."""

@@ -61,7 +61,7 @@ def check_pair(ctx, P, Q):
     assert ve == O.vefd(P, Q, lb)
 
 
-@pytest.mark.parametrize("cfg", [21, 42, 62, 63, 72, 73, 77, 83, 84, 107, 136, 137])
+@pytest.mark.parametrize("cfg", [21, 42, 62, 63, 72, 73, 77, 83, 84, 107, 136, 137, 1012, 1310, 1313])
 @pytest.mark.parametrize("dims", [2, 3])
 def test_fused_configs_bit_exact(contexts, cfg, dims):
     ctx = contexts(RFD_BOUNDS=cfg)
