@@ -228,6 +228,9 @@ def main() -> int:
 
     m = n = args.points
     P, Q = W.long_3d(m, n, seed=0)
+    sha = [W.sha256(P), W.sha256(Q)]
+    with open(os.path.join(ROOT, "tests", "golden", "workload_sha256.json")) as f:
+        want = json.load(f).get(f"long_3d_{m}x{n}_seed0")
     dims = 3
     ctx = Context.default()
     sess = Intersection(P, Q, ctx=ctx)
@@ -271,7 +274,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"3D random walks {m:,} x {n:,} points, N(0,1) steps, seed 0; "
                                f"DFD + VE-FD + bisection to 1e-9 relative, {args.precision}",
-                   "points": [m, n], "dims": dims, "stop": "rel 1e-9", "precision": args.precision},
+                   "points": [m, n], "dims": dims, "stop": "rel 1e-9", "precision": args.precision,
+                   "inputs_sha256": sha, "inputs_match_fixture": (sha == want) if want else None},
         "roofline": roof,
         "result": {"distance": results[-1]["distance"], "iterations": results[-1]["iterations"],
                    "dfd": results[-1]["dfd"], "vefd": results[-1]["vefd"],
