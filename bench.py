@@ -77,33 +77,83 @@ def measured_traffic(m: int, n: int, kernel: str):
     return None
 
 
-def cpu_baseline(P, Q, mix: dict, dims: int, budget_s: float = 12.0) -> dict:
-    """Times the CPU oracle (single thread) on row samples of each sweep the
-    GPU ran, and prices the GPU's per-kernel cell counts at those rates."""
+def cpu_threads() -> int:
+    """Host threads for the multi-core CPU timing: the cores this process may
+    run on, at most 16."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def cpu_model() -> str:
+    """The host CPU's model name as /proc/cpuinfo gives it."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _time_rows(run, rows0: int, rows_max: int, target_s: float, step: int = 1):
+    """Grows a row sample until one run takes about target_s; returns
+    (rows, seconds)."""
+    rows = max(step, rows0 - rows0 % step)
+    while True:
+        t0 = time.perf_counter()
+        run(rows)
+        dt = time.perf_counter() - t0
+        if dt > target_s or rows >= rows_max:
+            return rows, dt
+        grow = max(2.0, min(8.0, target_s / max(dt, 1e-4)))
+        rows = min(rows_max, int(rows * grow))
+        rows = max(step, rows - rows % step) if rows < rows_max else rows
+
+
+def cpu_baseline(P, Q, mix: dict, dims: int, budget_s: float = 20.0) -> dict:
+    """Times the CPU oracle on row samples of each sweep the GPU ran, on one
+    thread and on the host's cores (row blocks pipelined over column chunks,
+    bit-identical results), and prices the GPU run's per-kernel cell counts
+    at those rates.  The multi-core rate is the reported value; the decision
+    sweep, which the multi-core oracle does not split, is priced at its
+    single-thread rate times the thread count measured on the VE-FD sweep."""
     from oracle import oracle as O
 
     n = len(Q)
-    rates = {}
+    threads = cpu_threads()
     kinds = {"vefd_f64": 0, "vefd_f32": 0, "decide_f64": 1, "decide_f32": 1, "dfd": 2}
-    per = budget_s / max(1, len({kinds[k] for k in mix if k in kinds}))
+    used = sorted({kinds[k] for k in mix if k in kinds})
+    per = budget_s / max(1, 2 * len(used))
     lb = O.endpoint_lb(P, Q)
-    samples = []
-    for kind in sorted({kinds[k] for k in mix if k in kinds}):
-        rows = 8
-        while True:
-            t0 = time.perf_counter()
-            O.sweep_rows(P, Q, kind, lb * 1.5, 0, rows)
-            dt = time.perf_counter() - t0
-            if dt > per / 4 or rows >= len(P) - 1:
-                break
-            rows = min(len(P) - 1, int(rows * max(2.0, min(8.0, per / 4 / max(dt, 1e-4)))))
-        cells = rows * (n - 1 if kind != 2 else n)
-        rates[kind] = cells / dt
-        samples.append(f"{['vefd', 'decide', 'dfd'][kind]}: rows 0..{rows} x {n} cols")
-    t_est = sum(mix[k]["cells"] / rates[kinds[k]] for k in mix if k in kinds)
+    rate1, rate_mt, samples = {}, {}, []
+    for kind in used:
+        rmax = len(P) - 1 if kind != 2 else len(P)
+        cols = n - 1 if kind != 2 else n
+        rows, dt = _time_rows(lambda r: O.sweep_rows(P, Q, kind, lb * 1.5, 0, r), 8, rmax, per / 4)
+        rate1[kind] = rows * cols / dt
+        name = ["vefd", "decide", "dfd"][kind]
+        samples.append(f"{name}: rows 0..{rows} x {cols} cols on 1 thread")
+        if kind != 1 and threads > 1:
+            rows, dt = _time_rows(lambda r: O.sweep_rows_mt(P, Q, kind, lb * 1.5, 0, r, threads),
+                                  8 * threads, rmax, per / 4, step=threads)
+            rate_mt[kind] = rows * cols / dt
+            samples.append(f"{name}: rows 0..{rows} x {cols} cols on {threads} threads")
+    if 1 in used:
+        speed = rate_mt.get(0, rate1.get(0, 1.0)) / rate1.get(0, 1.0) if 0 in rate1 else threads
+        rate_mt[1] = rate1[1] * speed
+    for k in rate1:
+        rate_mt.setdefault(k, rate1[k])
     total = sum(mix[k]["cells"] for k in mix if k in kinds)
-    return {"value": total / t_est, "unit": "cells/s", "cores": 1, "kind": "port",
-            "sample": "CPU oracle (g++ -O3, FP64, one thread) on " + "; ".join(samples) +
+    t1 = sum(mix[k]["cells"] / rate1[kinds[k]] for k in mix if k in kinds)
+    tm = sum(mix[k]["cells"] / rate_mt[kinds[k]] for k in mix if k in kinds)
+    return {"value": total / tm, "unit": "cells/s", "cores": threads, "kind": "port",
+            "single_thread_value": total / t1, "cpu_model": cpu_model(),
+            "host_cpus": os.cpu_count(),
+            "sample": "CPU oracle (g++ -O3, FP64) on " + "; ".join(samples) +
                       "; rates applied to the GPU run's per-sweep cell counts"}
 
 

@@ -24,6 +24,8 @@
 #include <cstdlib>
 #include <vector>
 #include <algorithm>
+#include <atomic>
+#include <thread>
 
 namespace {
 
@@ -379,6 +381,119 @@ double or_sweep_rows(const double *P, int64_t m, const double *Q, int64_t n, int
         return s;
     }
     return dfd_impl<double>(P + r0 * dims, r1 - r0, Q, n, dims);
+}
+
+// Row-block pipeline of the bound sweeps for the multi-core timing: thread t
+// owns rows [r0 + t*B, ...) and walks the columns in chunks; before chunk c it
+// waits until thread t-1 has published its last row's outputs for chunk c.
+// Per cell the arithmetic is that of vefd_rows / dfd_impl, in the same order.
+namespace {
+
+struct alignas(128) Progress { std::atomic<int64_t> v{0}; };
+
+struct BlockJob {
+    const double *P, *Q;
+    int D, kind;
+    double r, big;
+    int64_t a, b, cols, chunk;
+    const double *edge_in;     // previous block's last-row outputs, or null for the first block
+    const Progress *ready_in;  // columns of edge_in already published
+    double *edge_out;
+    Progress *ready_out;
+    double *last;              // DFD: top of the block's last cell
+};
+
+// One row of one column chunk.  `left` is read from lin (the row above), the
+// row's outputs go to lout (lin == lout is allowed: each column is read
+// before it is written).
+inline double row_chunk(const BlockJob &J, int64_t i, int64_t c0, int64_t c1, double bt,
+                        const double *lin, double *lout) {
+    const int D = J.D;
+    const double *pa = J.P + i * D, *pb = pa + D;
+    if (J.kind == 0) {
+        for (int64_t j = c0; j < c1; ++j) {
+            const double *qa = J.Q + j * D, *qb = qa + D;
+            const double in = std::fmin(lin[j - c0], bt);
+            const double wr = point_seg2(pb, qa, qb, D);
+            const double wt = point_seg2(qb, pa, pb, D);
+            bt = std::fmax(wt, in);
+            lout[j - c0] = std::fmax(wr, in);
+        }
+    } else {
+        for (int64_t j = c0; j < c1; ++j) {
+            const double d = dist2(pa, J.Q + j * D, D);
+            const double in = std::fmin(lin[j - c0], bt);
+            const double top = std::fmax(d, in);
+            lout[j - c0] = std::fmin(top, bt);
+            bt = top;
+        }
+    }
+    return bt;
+}
+
+void run_block(BlockJob J) {
+    if (J.a >= J.b) {
+        J.ready_out->v.store(J.cols, std::memory_order_release);
+        return;
+    }
+    const int64_t rows = J.b - J.a;
+    std::vector<double> bot((size_t)rows);
+    for (int64_t i = J.a; i < J.b; ++i)
+        bot[(size_t)(i - J.a)] = (J.kind == 0 && i == 0) ? J.r * J.r : J.big;
+    std::vector<double> buf((size_t)J.chunk), first((size_t)J.chunk);
+    for (int64_t c0 = 0; c0 < J.cols; c0 += J.chunk) {
+        const int64_t c1 = std::min(J.cols, c0 + J.chunk);
+        if (J.edge_in) {
+            while (J.ready_in->v.load(std::memory_order_acquire) < c1) std::this_thread::yield();
+            std::copy(J.edge_in + c0, J.edge_in + c1, first.begin());
+        } else {
+            // vefd_rows: the first row enters with left = big; dfd_impl on
+            // P + r0: (r0, 0) enters with 0, the rest of the row with big.
+            std::fill(first.begin(), first.begin() + (c1 - c0), J.big);
+            if (J.kind == 2 && c0 == 0) first[0] = 0.0;
+        }
+        bot[0] = row_chunk(J, J.a, c0, c1, bot[0], first.data(), buf.data());
+        for (int64_t i = J.a + 1; i < J.b; ++i)
+            bot[(size_t)(i - J.a)] = row_chunk(J, i, c0, c1, bot[(size_t)(i - J.a)], buf.data(), buf.data());
+        std::copy(buf.begin(), buf.begin() + (c1 - c0), J.edge_out + c0);
+        J.ready_out->v.store(c1, std::memory_order_release);
+    }
+    *J.last = bot[(size_t)rows - 1];
+}
+
+}  // namespace
+
+double or_sweep_rows_mt(const double *P, int64_t m, const double *Q, int64_t n, int dims,
+                        int kind, double r, int64_t r0, int64_t r1, int threads) {
+    if ((kind != 0 && kind != 2) || r1 <= r0) return or_sweep_rows(P, m, Q, n, dims, kind, r, r0, r1);
+    const int64_t cols = (kind == 0) ? n - 1 : n;
+    const int64_t rows = r1 - r0;
+    if (threads < 1) threads = 1;
+    if (threads > rows) threads = (int)std::max<int64_t>(rows, 1);
+    const int64_t B = (rows + threads - 1) / threads;
+    threads = (int)((rows + B - 1) / B);  // every block non-empty
+    std::vector<std::vector<double>> edge((size_t)threads, std::vector<double>((size_t)cols));
+    std::vector<Progress> ready((size_t)threads);
+    std::vector<double> last((size_t)threads * 16, 0.0);  // one cache line apart
+    std::vector<BlockJob> jobs((size_t)threads);
+    for (int t = 0; t < threads; ++t) {
+        BlockJob &J = jobs[(size_t)t];
+        J.P = P; J.Q = Q; J.D = dims; J.kind = kind; J.r = r; J.big = (double)FLT_MAX;
+        J.a = r0 + t * B; J.b = std::min(r1, J.a + B); J.cols = cols; J.chunk = 256;
+        J.edge_in = t > 0 ? edge[(size_t)t - 1].data() : nullptr;
+        J.ready_in = t > 0 ? &ready[(size_t)t - 1] : nullptr;
+        J.edge_out = edge[(size_t)t].data();
+        J.ready_out = &ready[(size_t)t];
+        J.last = &last[(size_t)t * 16];
+    }
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(run_block, jobs[(size_t)t]);
+    run_block(jobs[0]);
+    for (auto &th : pool) th.join();
+    if (kind == 2) return std::sqrt(last[(size_t)(threads - 1) * 16]);
+    double s = 0;
+    for (double v : edge[(size_t)threads - 1]) s += v;
+    return s;
 }
 
 }  // extern "C"

@@ -73,6 +73,13 @@ int64_t or_distance(const double *P, int64_t m, const double *Q, int64_t n, int 
 double or_sweep_rows(const double *P, int64_t m, const double *Q, int64_t n, int dims,
                      int kind, double r, int64_t r0, int64_t r1);
 
+/* The same bound sweeps (kind 0 = VE-FD, 2 = DFD) over rows [r0, r1) on
+ * `threads` CPU threads: contiguous row blocks, one per thread, pipelined in
+ * column chunks (block t consumes block t-1's last-row outputs chunk by
+ * chunk), for the multi-core CPU timing.  Bit-identical to or_sweep_rows. */
+double or_sweep_rows_mt(const double *P, int64_t m, const double *Q, int64_t n, int dims,
+                        int kind, double r, int64_t r0, int64_t r1, int threads);
+
 #ifdef __cplusplus
 }
 #endif

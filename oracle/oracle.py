@@ -52,6 +52,9 @@ def lib():
         L.or_sweep_rows.argtypes = [_dp, _i64, _dp, _i64, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_double, _i64, _i64]
         L.or_sweep_rows.restype = ctypes.c_double
+        L.or_sweep_rows_mt.argtypes = [_dp, _i64, _dp, _i64, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_double, _i64, _i64, ctypes.c_int]
+        L.or_sweep_rows_mt.restype = ctypes.c_double
         _LIB = L
     return _LIB
 
@@ -141,3 +144,11 @@ def distance(P, Q, stop_mode: int = 1, eps: float = 1e-9, vefd_gate: bool = True
 def sweep_rows(P, Q, kind: int, r: float, r0: int, r1: int) -> float:
     P, Q = _c(P), _c(Q)
     return lib().or_sweep_rows(_p(P), len(P), _p(Q), len(Q), P.shape[1], kind, r, r0, r1)
+
+
+def sweep_rows_mt(P, Q, kind: int, r: float, r0: int, r1: int, threads: int) -> float:
+    """sweep_rows on `threads` CPU threads (row blocks pipelined over column
+    chunks); kinds 0 (VE-FD) and 2 (DFD), same result as sweep_rows."""
+    P, Q = _c(P), _c(Q)
+    return lib().or_sweep_rows_mt(_p(P), len(P), _p(Q), len(Q), P.shape[1], kind, r, r0, r1,
+                                  threads)

@@ -83,6 +83,12 @@ def vefd_brute(m: int, n: int, weights, init: float) -> float:
 
 def vefd_readme(m: int, n: int, weights, init: float) -> float:
     """T/R recurrence of README.md:97-126 over full matrices, squared values."""
+    T, Rr = vefd_readme_tables(m, n, weights, init)
+    return float(min(T[-1, -1], Rr[-1, -1]))
+
+
+def vefd_readme_tables(m: int, n: int, weights, init: float):
+    """The full T and R tables of the README.md:97-126 recurrence."""
     R, C = m - 1, n - 1
     T = np.full((R, C), np.inf)
     Rr = np.full((R, C), np.inf)
@@ -94,7 +100,7 @@ def vefd_readme(m: int, n: int, weights, init: float) -> float:
             wt, wr = weights(i, j)
             T[i, j] = max(wt, inn)
             Rr[i, j] = max(wr, inn)
-    return float(min(T[-1, -1], Rr[-1, -1]))
+    return T, Rr
 
 
 def point_seg_dist(x, a, b) -> float:

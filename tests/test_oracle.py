@@ -308,3 +308,38 @@ def test_fp32_forms_track_fp64():
         if abs(r - ub) / ub < 1e-4:
             continue
         assert O.decide(P, Q, r, fp32=True) == O.decide(P, Q, r)
+
+
+# ---------------- timing sweeps (CPU baseline) ----------------
+
+@pytest.mark.parametrize("seed", range(3))
+def test_sweep_rows_pinned(seed):
+    """The timing sweeps compute the oracle's recurrences: the DFD kind returns
+    DFD of the sampled rows against Q, the VE-FD kind the sum of the last
+    sampled row's right-edge values of the README recurrence."""
+    rng = np.random.default_rng(200 + seed)
+    m, n = 23 + seed, 31 - seed
+    P, Q = walk(rng, m, 3), walk(rng, n, 3)
+    assert O.sweep_rows(P, Q, 2, 0.0, 0, m) == pytest.approx(I.dfd_eiter_mannila(P, Q), rel=4e-16)
+    assert O.sweep_rows(P, Q, 2, 0.0, 5, 17) == pytest.approx(
+        I.dfd_eiter_mannila(P[5:17], Q), rel=4e-16)
+    init = O.endpoint_lb(P, Q) * 1.3
+    _, Rr = I.vefd_readme_tables(m, n, lambda i, j: O.ve_weights(P, Q, i, j), init)
+    for rows in (1, 7, m - 1):
+        assert O.sweep_rows(P, Q, 0, init, 0, rows) == pytest.approx(Rr[rows - 1].sum(), rel=1e-15)
+
+
+@pytest.mark.parametrize("kind", [0, 2])
+@pytest.mark.parametrize("threads", [1, 2, 3, 8, 40])
+def test_sweep_rows_mt_matches_single_thread(kind, threads):
+    """The multi-core timing sweep is bit-identical to the single-thread one:
+    ragged row blocks, more threads than rows, column counts that are not a
+    multiple of the 256-column chunk, samples that do not start at row 0."""
+    rng = np.random.default_rng(300 + threads)
+    P, Q = walk(rng, 61, 3), walk(rng, 700, 3)
+    r = O.endpoint_lb(P, Q) * 2.0
+    rmax = len(P) - 1 if kind == 0 else len(P)
+    for r0, r1 in ((0, rmax), (0, 1), (0, 17), (9, 40), (rmax - 3, rmax)):
+        st = O.sweep_rows(P, Q, kind, r, r0, r1)
+        mt = O.sweep_rows_mt(P, Q, kind, r, r0, r1, threads)
+        assert mt == st, (r0, r1)
