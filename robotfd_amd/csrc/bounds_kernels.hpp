@@ -28,9 +28,10 @@
 // them (the anti-diagonal of band_sweep).  Producer wave w computes, for every
 // lane and step, the weights of rows t = w, w + NPW, ... (< R) into slot s mod KS
 // of the ring: three doubles per cell (DFD distance^2, VE-FD right and top
-// weights), lane-contiguous.  Producers take q_{s-l} from lane l-1's previous
-// point by a DPP shift, with lane 0 fed from a register queue of the next 64
-// points of Q (one coalesced load per 64 steps, issued a chunk ahead).
+// weights), lane-contiguous.  Producers read q_{s-l} from a ring of Q columns
+// in LDS (each wave stores every 64-column chunk itself, one chunk ahead, from
+// a coalesced load issued a chunk before that), and carry q_{j-1} and the
+// differences p_i - q_{j-1} from the step before.
 //
 // Hand-offs.  Inside the workgroup: producer w publishes the count of steps it
 // has written (LDS word, after s_waitcnt lgkmcnt(0)); the chain publishes the
@@ -63,11 +64,11 @@ struct BoundsArgs {
 __device__ inline int lds_get(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 __device__ inline void lds_put(int *p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 
-// Ring depth in steps, 1536 R bytes each: about 48 KB per workgroup for the
-// configurations with few producers (several workgroups per CU), about 120 KB
-// for those with five or more (one workgroup per CU).
+// Ring depth in steps, 1536 R bytes each: at most 43 KB per workgroup for the
+// configurations with few producers (three workgroups per CU with the other
+// rings), about 120 KB for those with five or more (one workgroup per CU).
 template <int R, int NPW> struct BoundsRing {
-    static constexpr int budget = (NPW < 5) ? 49152 : 122880;
+    static constexpr int budget = (NPW < 5) ? 44032 : 122880;
     static constexpr int KS = (budget / (1536 * R) < 16) ? budget / (1536 * R) : 16;
     static_assert(KS >= 2, "ring needs two slots");
 };
@@ -116,6 +117,28 @@ __device__ inline double seg_w(const double *x, const double *a, const double *a
     return w;
 }
 
+// seg_w with the difference x - a given (ap = x - a, as seg_w forms it).
+template <int D>
+__device__ inline double seg_w_ap(const double *x, const double *a, const double *ap, const double *ab,
+                                  double ab2) {
+    double apab = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) apab = __builtin_fma(ap[k], ab[k], apab);
+    double t = (apab > 0.0) ? 1.0 : 0.0;
+    if (apab > 0.0 && apab < ab2) {
+        asm volatile("");  // keep the divide in a branch, not a select
+        t = apab / ab2;
+        t = __builtin_fmin(1.0, __builtin_fmax(0.0, t));
+    }
+    double w = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double d = x[k] - __builtin_fma(t, ab[k], a[k]);
+        w = __builtin_fma(d, d, w);
+    }
+    return w;
+}
+
 // min / max of doubles that are never NaN: a bare v_min_f64 / v_max_f64.
 // __builtin_fmin would first quiet each operand the compiler cannot prove
 // canonical (LDS loads, DPP results, loop-carried values), one extra
@@ -131,46 +154,48 @@ __device__ inline double dmax(double a, double b) {
     return r;
 }
 
-// R cells of one chain lane at one step.  All 3R weights are read before the
-// first min/max so that one LDS round trip, not R of them, sits on the chain.
-//
-// Per row, with dl / vl the values from the row below and dbot / vbot the
-// row's own values from the previous column:
-//   DFD   (.cu:458-641)  top = max(d, min(dl, dbot)),  dl' = min(top, dbot)
-//   VE-FD (.cu:200-432)  in = min(vl, vbot),  T = max(wt, in),  vl' = R = max(wr, in)
-// Rows G at a time (one LDS round trip per group): G = R up to 8 rows per
-// lane, half of R above, which bounds the registers the weights hold.
-template <int R, int T0, int G>
-__device__ inline void chain_group(const double *w, double (&dbot)[R], double (&vbot)[R], double &dl,
-                                   double &vl) {
-    double wv[G][3];
+// Weights of rows T0 .. T0 + G - 1 of one chain lane at one step, issued as
+// one batch of LDS loads.
+template <int T0, int G>
+__device__ inline void chain_load(const double *w, double (&wv)[G][3]) {
 #pragma unroll
     for (int g = 0; g < G; ++g)
 #pragma unroll
         for (int c = 0; c < 3; ++c) wv[g][c] = w[((T0 + g) * 3 + c) * 64];
+}
+
+// Orders a group's results before any later memory access: the next step's
+// loads of the group's weights must not be hoisted above the compute that
+// frees their registers.
+template <int R, int T0, int G>
+__device__ inline void chain_done(const double (&dbot)[R], const double (&vbot)[R], double dl, double vl) {
+    asm volatile("" ::"v"(dl), "v"(vl) : "memory");
 #pragma unroll
-    for (int g = 0; g < G; ++g)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) asm volatile("" ::"v"(wv[g][c]));
+    for (int g = 0; g < G; ++g) asm volatile("" ::"v"(dbot[T0 + g]), "v"(vbot[T0 + g]) : "memory");
+}
+
+// Rows T0 .. T0 + G - 1 of one chain lane at one step.  Per row, with dl / vl
+// the values from the row below and dbot / vbot the row's own values from the
+// previous column:
+//   DFD   (.cu:458-641)  top = max(d, min(dl, dbot)),  dl' = min(top, dbot)
+//   VE-FD (.cu:200-432)  in = min(vl, vbot),  T = max(wt, in),  vl' = R = max(wr, in)
+template <int R, int T0, int G>
+__device__ inline void chain_rows(const double (&wv)[G][3], double (&dbot)[R], double (&vbot)[R], double &dl,
+                                  double &vl) {
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         const int t = T0 + g;
         const double d = wv[g][0], wr = wv[g][1], wt = wv[g][2];
-        const double top = dmax(d, dmin(dl, dbot[t]));
-        dl = dmin(top, dbot[t]);
-        dbot[t] = top;
+        // dl' = min(top, dbot) = min(max(d, dl), dbot) (min distributes over
+        // max, and these doubles are never NaN): two operations on the serial
+        // path instead of three, and no copy of the old dbot.
+        const double dn = dmin(dmax(d, dl), dbot[t]);
+        dbot[t] = dmax(d, dmin(dl, dbot[t]));
+        dl = dn;
         const double vin = dmin(vl, vbot[t]);
         vbot[t] = dmax(wt, vin);
         vl = dmax(wr, vin);
     }
-}
-
-template <int R>
-__device__ inline void chain_cells(const double *w, double (&dbot)[R], double (&vbot)[R], double &dl,
-                                   double &vl) {
-    constexpr int G = (R <= 8) ? R : (R + 1) / 2;
-    chain_group<R, 0, G>(w, dbot, vbot, dl, vl);
-    if constexpr (G < R) chain_group<R, G, R - G>(w, dbot, vbot, dl, vl);
 }
 
 // Columns of the band-boundary rings in LDS (incoming and outgoing).
@@ -190,9 +215,11 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, const d
                                     const double *inring, double *outring, const BoundsSync &sy,
                                     const int npw) {
     const int lane = (int)(threadIdx.x & 63);
-    const long long rows = a.rows, cols = a.cols;
     const double big = (double)FLT_MAX;
-    const long long i0 = (long long)band * 64 * R + (long long)R * lane;
+    // Steps and columns in 32 bits (the launch requires n < 2^31 - 64), so
+    // that the loop's bookkeeping stays on the scalar unit.
+    const int cols = (int)a.cols;
+    const int nsteps = cols + 63;
     double dbot[R], vbot[R];
 #pragma unroll
     for (int t = 0; t < R; ++t) {
@@ -202,29 +229,45 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, const d
     double dout = big, vout = big;
     const bool has_pred = band > 0;
     const bool has_succ = band + 1 < a.nbands;
-    const long long nsteps = cols + 63;
-    const long long tl = rows - 1 - i0;  // the terminal row's offset in this lane (if in [0, R))
-    long long avail = 0, in_known = 0, done_known = 0;
+    // Terminal cell (m-1, n-1): when this band holds row m - 1, it is row tl
+    // of lane tlane, which reaches column n - 1 at step n - 1 + tlane.
+    const long long toff = a.rows - 1 - (long long)band * 64 * R;
+    const bool has_term = toff >= 0 && toff < 64LL * R;
+    const int tlane = has_term ? (int)(toff / R) : -1;
+    const int tl = has_term ? (int)(toff - (long long)tlane * R) : 0;
+    const int s_term = has_term ? cols - 1 + tlane : -1;
+    int avail = 0, in_known = 0, done_known = 0;
     int slot = 0;
     if (a.tlog && lane == 0) a.tlog[4 * band] = wall_clock64();
     // The chain is the band's serial path and every later band waits on it:
     // its SIMD issues for it first, producers fill the remaining cycles.
     __builtin_amdgcn_s_setprio(3);
 
-    for (long long s = 0; s < nsteps; ++s) {
-        const long long j = s - lane;
-        // Row 64Rb - 1 at column s (lane 0's input from below), from the ring
-        // the courier fills.
-        if (has_pred && s < cols && s >= in_known) {
+    // Software pipeline over three row groups A, B, C: a group's LDS loads
+    // are issued while the group before it computes (B's and C's during the
+    // same step, A's and the input from below during C of the step before),
+    // so no LDS round trip sits between two steps' min/max chains, and at most
+    // two groups' weights occupy registers at a time.  Loads are issued on
+    // every path (the last step reads a slot and a ring entry nobody uses), so
+    // that the compiler's LDS counter waits stay exact.  Polls, which wait for
+    // all of this wave's LDS operations, come right after a group's compute.
+    constexpr int GA = (R + 2) / 3, GB = (R - GA + 1) / 2, GC = R - GA - GB;
+    double wa[GA][3], wb[GB > 0 ? GB : 1][3], wc[GC > 0 ? GC : 1][3];
+    double bD, bV;  // the courier's ring entries for the next step's input from below
+    auto wait_pred = [&](const int st) __attribute__((always_inline)) {
+        // Row 64Rb - 1 at column st (lane 0's input from below).
+        if (has_pred && st < cols && st >= in_known) {
             for (;;) {
                 in_known = __builtin_amdgcn_readfirstlane(lds_get(sy.in_avail));
-                if (s < in_known) break;
+                if (st < in_known) break;
                 __builtin_amdgcn_s_sleep(1);
             }
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            if (a.tlog && lane == 0 && s == 0) a.tlog[4 * band + 1] = wall_clock64();
+            if (a.tlog && lane == 0 && st == 0) a.tlog[4 * band + 1] = wall_clock64();
         }
-        if (s >= avail) {
+    };
+    auto wait_prod = [&](const int st) __attribute__((always_inline)) {
+        if (st >= avail) {
             for (;;) {
                 int mn = lds_get(&sy.prod[0]);
                 for (int w = 1; w < npw; ++w) {
@@ -232,39 +275,90 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, const d
                     mn = v < mn ? v : mn;
                 }
                 avail = __builtin_amdgcn_readfirstlane(mn);
-                if (s < avail) break;
+                if (st < avail) break;
                 __builtin_amdgcn_s_sleep(1);
             }
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
         }
-        const double *w = wring + (long long)slot * (R * 3 * 64) + lane;
-        double dl = shift_up(dout), vl = shift_up(vout);
-        double bD, bV;
-        if (has_pred) {
-            const int r = (int)((s < cols ? s : cols - 1) % kRing);
-            bD = inring[r];
-            bV = inring[kRing + r];
-        } else {
-            bD = (s == 0) ? 0.0 : big;
-            bV = big;
-        }
-        if (lane == 0) {
-            dl = bD;
-            vl = bV;
-        }
-        if (j >= 0 && j < cols) {  // every lane, except while the band fills and drains
-            chain_cells<R>(w, dbot, vbot, dl, vl);
-            dout = dl;
-            vout = vl;
-        }
+    };
+    auto load_in = [&](const int st) __attribute__((always_inline)) {
+        const int r = (st < cols ? st : cols - 1) & (kRing - 1);
+        bD = inring[r];
+        bV = inring[kRing + r];
+    };
+    wait_pred(0);
+    wait_prod(0);
+    load_in(0);
+    chain_load<0, GA>(wring + lane, wa);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+
+    // Every lane runs every step: before its first column the producers'
+    // weights keep its rows at their initial values, and after its last
+    // column it computes values nobody reads (the terminal cell is written at
+    // its own step, and lane 63's hand-off column never passes n - 1).
+    for (int s = 0; s < nsteps; ++s) {
+        const int nslot = (slot + 1 == KS) ? 0 : slot + 1;
+        const double *wcur = wring + slot * (R * 3 * 64) + lane;
+        const double *wnext = wring + nslot * (R * 3 * 64) + lane;
+        if constexpr (GB > 0) chain_load<GA, GB>(wcur, wb);
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if (lane == 0) lds_put(sy.cons, (int)(s + 1));
-        slot = (slot + 1 == KS) ? 0 : slot + 1;
+        // Input from below: lane l-1's outputs of the step before, and for
+        // lane 0 the row under the band (the first band's row -1 holds DFD's
+        // 0 at column 0 and +big after it, VE-FD's +big).
+        double dl = shift_up(dout), vl = shift_up(vout);
+        const double inD = has_pred ? bD : (s == 0 ? 0.0 : big);
+        const double inV = has_pred ? bV : big;
+        if (lane == 0) {
+            dl = inD;
+            vl = inV;
+        }
+        chain_rows<R, 0, GA>(wa, dbot, vbot, dl, vl);
+        chain_done<R, 0, GA>(dbot, vbot, dl, vl);
+        if constexpr (GC > 0) chain_load<GA + GB, GC>(wcur, wc);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if constexpr (GB > 0) {
+            chain_rows<R, GA, GB>(wb, dbot, vbot, dl, vl);
+            chain_done<R, GA, GB>(dbot, vbot, dl, vl);
+        }
+        if (s + 1 < nsteps) {
+            wait_prod(s + 1);
+            wait_pred(s + 1);
+        }
+        chain_load<0, GA>(wnext, wa);
+        load_in(s + 1);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if constexpr (GC > 0) {
+            chain_rows<R, GA + GB, GC>(wc, dbot, vbot, dl, vl);
+            chain_done<R, GA + GB, GC>(dbot, vbot, dl, vl);
+        }
+        dout = dl;
+        vout = vl;
+        if (s == s_term) {
+            // Terminal cell: its DFD value is the row's top; its VE-FD value
+            // is min(T, R), where R left row tl and, rows above tl being
+            // padding that passes VE-FD values through, is vout.
+            if (lane == tlane) {
+                double topD = dbot[0], topV = vbot[0];
+#pragma unroll
+                for (int t = 1; t < R; ++t) {
+                    topD = (tl == t) ? dbot[t] : topD;
+                    topV = (tl == t) ? vbot[t] : topV;
+                }
+                const double resV = dmin(topV, vout);
+                a.result[0] = __builtin_sqrt(topD);
+                a.result[1] = topD;
+                a.result[2] = __builtin_sqrt(resV);
+                a.result[3] = resV;
+            }
+        }
+        // Step s's weights are all in registers: its ring slot is free.
+        if (lane == 0) lds_put(sy.cons, s + 1);
+        slot = nslot;
 
         // Row 64R(b+1) - 1 at column s - 63 (lane 63's output) into the
         // outgoing ring, for the courier to publish.
         if (has_succ && s >= 63) {
-            const long long col = s - 63;
+            const int col = s - 63;
             if (col >= done_known + kRing) {
                 for (;;) {
                     done_known = __builtin_amdgcn_readfirstlane(lds_get(sy.out_done));
@@ -274,31 +368,15 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, const d
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
             }
             if (lane == 63) {
-                const int r = (int)(col % kRing);
+                const int r = col & (kRing - 1);
                 outring[r] = dout;
                 outring[kRing + r] = vout;
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                lds_put(sy.out_avail, (int)(col + 1));
+                lds_put(sy.out_avail, col + 1);
             }
         }
     }
     __builtin_amdgcn_s_setprio(0);
-    // Terminal cell (m-1, n-1): its DFD value is the row's last top; its VE-FD
-    // value is min(T, R), where R left row tl at the last column and, rows
-    // above tl being padding that passes VE-FD values through, is vout.
-    if (tl >= 0 && tl < R) {
-        double topD = dbot[0], topV = vbot[0];
-#pragma unroll
-        for (int t = 1; t < R; ++t) {
-            topD = (tl == t) ? dbot[t] : topD;
-            topV = (tl == t) ? vbot[t] : topV;
-        }
-        const double resV = dmin(topV, vout);
-        a.result[0] = __builtin_sqrt(topD);
-        a.result[1] = topD;
-        a.result[2] = __builtin_sqrt(resV);
-        a.result[3] = resV;
-    }
     if (a.tlog && lane == 0) {
         a.tlog[4 * band + 2] = wall_clock64();
         a.tlog[4 * band + 3] = ((unsigned long long)__builtin_amdgcn_s_getreg((20 << 0) | (0 << 6) | (31 << 11)) << 32) |
@@ -375,9 +453,13 @@ __device__ inline void bounds_courier(const BoundsArgs &a, const int band, doubl
     }
 }
 
+// Columns of Q staged in LDS for the producers: four 64-column chunks,
+// component-major (component k of column j at k * kQRing + (j & (kQRing - 1))).
+constexpr int kQRing = 256;
+
 template <int D, int R, int NPW, int KS>
 __device__ inline void bounds_producer(const BoundsArgs &a, const int band, const int pw, double *wring,
-                                       int *prod, int *cons) {
+                                       double *qring, int *prod, int *cons) {
     // Rows t = pw, pw + NPW, ... (< R) of every lane.
     constexpr int RP = (R + NPW - 1) / NPW;
     const int nmine = (R - pw + NPW - 1) / NPW;
@@ -404,41 +486,57 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         }
         ab2[k] = acc;
     }
-    // Q queue: lane l holds point s0 + l of the current 64-point chunk, shifted
-    // down one lane per step so lane 0 holds q_s at step s.
-    double qq[D], qn[D], qa[D], qb[D];
+    // Q ring.  Every producer wave stores each chunk itself (the same values),
+    // chunk c + 1 at its step 64c, from registers loaded one chunk earlier.
+    // A wave at step s reads columns s - 63 .. s; producers stay within KS
+    // (< 64) steps of each other (the chain waits for the slowest, and each
+    // runs at most KS steps ahead of the chain), so no wave still reads chunk
+    // c - 3, whose slots chunk c + 1 takes, and none reads chunk c + 1 before
+    // its own store of it (a wave's LDS operations complete in order).
+    double qn[D];
     {
         const long long c0 = lane < cols ? lane : cols - 1;
+        const long long c1 = 64 + lane < cols ? 64 + lane : cols - 1;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            qq[k] = Q[c0 * D + k];
-            qn[k] = qq[k];
-            qb[k] = 0.0;
+            qring[k * kQRing + lane] = Q[c0 * D + k];
+            qn[k] = Q[c1 * D + k];
         }
     }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
     const double r2 = a.r2;
     const long long nsteps = cols + 63;
     long long cons_seen = 0;
     int slot = 0;
-    for (long long s = 0; s < nsteps; ++s) {
+    // Lane l at step s evaluates column j = s - l.  Carried from the step
+    // before: q_{j-1}, and p_i - q_{j-1} of each row, which is the right
+    // weight's p_i - a term (the same operands and operation as the DFD
+    // difference one step earlier, so the same value).  Values at j <= 0 feed
+    // only weights that the boundary rule below overwrites.
+    double qA[D], qB[D], xA[RP][D], xB[RP][D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) qA[c] = 0.0;
+#pragma unroll
+    for (int k = 0; k < RP; ++k)
+#pragma unroll
+        for (int c = 0; c < D; ++c) xA[k][c] = 0.0;
+
+    auto step = [&](const long long s, const double(&qa)[D], double(&qb)[D], const double(&xp)[RP][D],
+                    double(&xn)[RP][D]) __attribute__((always_inline)) {
         if ((s & 63) == 0) {
-            if (s > 0) {
+            const long long c = s + 64 + lane;  // chunk s / 64 + 1
+            long long cn = c + 64;              // chunk s / 64 + 2, stored 64 steps later
+            cn = cn < cols ? cn : cols - 1;
 #pragma unroll
-                for (int k = 0; k < D; ++k) qq[k] = qn[k];
+            for (int k = 0; k < D; ++k) {
+                qring[k * kQRing + (int)(c & (kQRing - 1))] = qn[k];
+                qn[k] = Q[cn * D + k];
             }
-            long long c = s + 64 + lane;  // the chunk after this one, used 64 steps later
-            c = c < cols ? c : cols - 1;
-#pragma unroll
-            for (int k = 0; k < D; ++k) qn[k] = Q[c * D + k];
-        } else {
-#pragma unroll
-            for (int k = 0; k < D; ++k) qq[k] = dpp_f64(qq[k], qq[k], 1);
         }
+        const long long j = s - lane;
+        const int qi = (int)(j & (kQRing - 1));
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-            qa[k] = qb[k];
-            qb[k] = dpp_f64(qq[k], qb[k], 0);  // q_{s-l}; lane 0 takes q_s from the queue
-        }
+        for (int k = 0; k < D; ++k) qb[k] = qring[k * kQRing + qi];
         // Column terms of segment [q_{j-1}, q_j].
         double cab[D], cab2 = 0.0;
 #pragma unroll
@@ -456,7 +554,6 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         }
         double *w = wring + (long long)slot * (R * 3 * 64) + lane;
         const bool edge = s < 64 || band == 0 || band + 1 == a.nbands;
-        const long long j = s - lane;
 #pragma unroll
         for (int k = 0; k < RP; ++k) {
             if (RP * NPW != R && k == RP - 1 && k >= nmine) break;
@@ -464,10 +561,10 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
             double d = 0.0;
 #pragma unroll
             for (int c = 0; c < D; ++c) {
-                const double x = pi[k][c] - qb[c];
-                d = __builtin_fma(x, x, d);
+                xn[k][c] = pi[k][c] - qb[c];
+                d = __builtin_fma(xn[k][c], xn[k][c], d);
             }
-            const double wr = seg_w<D>(pi[k], qa, cab, cab2);
+            const double wr = seg_w_ap<D>(pi[k], qa, xp[k], cab, cab2);
             const double wt = seg_w<D>(qb, pm[k], ab[k], ab2[k]);
             w[(t * 3 + 0) * 64] = d;
             w[(t * 3 + 1) * 64] = wr;
@@ -485,7 +582,14 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
                 const int t = pw + k * NPW;
                 const long long i = i0 + t;
                 const bool pad = i >= rows;  // padding rows of the last band pass VE-FD values up unchanged
-                if (j == 0 || i == 0 || pad) {
+                if (j < 0) {
+                    // Before the lane's first column: weights that leave the
+                    // chain's row state at its initial values (DFD +big,
+                    // VE-FD -big), so the chain needs no per-lane branch.
+                    w[(t * 3 + 0) * 64] = big;
+                    w[(t * 3 + 1) * 64] = -big;
+                    w[(t * 3 + 2) * 64] = -big;
+                } else if (j == 0 || i == 0 || pad) {
                     w[(t * 3 + 1) * 64] = pad ? -big : big;
                     w[(t * 3 + 2) * 64] = (!pad && j == 0 && i == 1) ? r2 : big;
                 }
@@ -496,13 +600,22 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         if (lane == 0) lds_put(&prod[pw], (int)(s + 1));
         slot = (slot + 1 == KS) ? 0 : slot + 1;
+    };
+    // Two steps per trip, the carried values alternating between the A and B
+    // registers instead of being copied.
+    long long s = 0;
+    for (; s + 1 < nsteps; s += 2) {
+        step(s, qA, qB, xA, xB);
+        step(s + 1, qB, qA, xB, xA);
     }
+    if (s < nsteps) step(s, qA, qB, xA, xB);
 }
 
 // LDS of one workgroup of the fused sweep.
 template <int R, int NPW> struct BoundsShared {
     static constexpr int KS = BoundsRing<R, NPW>::KS;
     double wring[KS * R * 3 * 64];
+    double qring[3 * kQRing];
     double inring[2 * kRing];
     double outring[2 * kRing];
     int sprod[NPW];
@@ -526,7 +639,8 @@ __device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsSh
     else if (wave == cw)
         bounds_courier<2>(a, band, sh.inring, sh.outring, sy);
     else
-        bounds_producer<D, R, NPW, KS>(a, band, wave - 1 - (wave > cw ? 1 : 0), sh.wring, sh.sprod, sy.cons);
+        bounds_producer<D, R, NPW, KS>(a, band, wave - 1 - (wave > cw ? 1 : 0), sh.wring, sh.qring, sh.sprod,
+                                       sy.cons);
 }
 
 template <int R, int NPW>
