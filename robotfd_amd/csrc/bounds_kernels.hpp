@@ -42,6 +42,8 @@
 // never waits on global memory.
 #pragma once
 
+#include <type_traits>
+
 #include "sweep_kernels.hpp"
 
 namespace rfd {
@@ -64,12 +66,13 @@ struct BoundsArgs {
 __device__ inline int lds_get(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 __device__ inline void lds_put(int *p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 
-// Ring depth in steps, 1536 R bytes each: at most 43 KB per workgroup for the
-// configurations with few producers (three workgroups per CU with the other
-// rings), about 120 KB for those with five or more (one workgroup per CU).
-template <int R, int NPW> struct BoundsRing {
+// Ring depth in steps, 1536 R C bytes each (C columns per step): at most 43 KB
+// per workgroup for the configurations with few producers (three workgroups
+// per CU with the other rings), about 120 KB for those with five or more (one
+// workgroup per CU).
+template <int R, int NPW, int C = 1> struct BoundsRing {
     static constexpr int budget = (NPW < 5) ? 44032 : 122880;
-    static constexpr int KS = (budget / (1536 * R) < 16) ? budget / (1536 * R) : 16;
+    static constexpr int KS = (budget / (1536 * R * C) < 16) ? budget / (1536 * R * C) : 16;
     static_assert(KS >= 2, "ring needs two slots");
 };
 
@@ -154,48 +157,78 @@ __device__ inline double dmax(double a, double b) {
     return r;
 }
 
-// Weights of rows T0 .. T0 + G - 1 of one chain lane at one step, issued as
-// one batch of LDS loads.
-template <int T0, int G>
-__device__ inline void chain_load(const double *w, double (&wv)[G][3]) {
+// Row groups of the chain's software pipeline: NG groups of at most GMAX
+// rows, group g holding rows start(g) .. start(g + 1) - 1.  Three groups at
+// one column per step; groups of at most three rows at two.
+template <int R, int C> struct ChainPlan {
+    static constexpr int GT = (C == 1) ? (R + 2) / 3 : 3;
+    static constexpr int NG = (R + GT - 1) / GT;
+    static constexpr int GMAX = (R + NG - 1) / NG;
+    static constexpr int start(int g) { return (R * g) / NG; }
+    static constexpr int size(int g) { return (R * (g + 1)) / NG - (R * g) / NG; }
+};
+
+// Weights of group g for the C columns of one step, issued as one batch of LDS
+// loads.  A slot holds [column][row][d, wr, wt][lane].
+template <int R, int C, int g>
+__device__ inline void chain_load(const double *w, double (&wv)[C][ChainPlan<R, C>::GMAX][3]) {
+    using PL = ChainPlan<R, C>;
 #pragma unroll
-    for (int g = 0; g < G; ++g)
+    for (int c = 0; c < C; ++c)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) wv[g][c] = w[((T0 + g) * 3 + c) * 64];
+        for (int i = 0; i < PL::size(g); ++i)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) wv[c][i][k] = w[(c * R + PL::start(g) + i) * 3 * 64 + k * 64];
 }
 
-// Orders a group's results before any later memory access: the next step's
-// loads of the group's weights must not be hoisted above the compute that
-// frees their registers.
-template <int R, int T0, int G>
-__device__ inline void chain_done(const double (&dbot)[R], const double (&vbot)[R], double dl, double vl) {
-    asm volatile("" ::"v"(dl), "v"(vl) : "memory");
+// Orders a group's results before any later memory access: the next loads
+// into the group's registers must not be hoisted above the compute that
+// frees them.
+template <int R, int C, int g>
+__device__ inline void chain_done(const double (&dbot)[R], const double (&vbot)[R], const double (&dl)[C],
+                                  const double (&vl)[C]) {
+    using PL = ChainPlan<R, C>;
 #pragma unroll
-    for (int g = 0; g < G; ++g) asm volatile("" ::"v"(dbot[T0 + g]), "v"(vbot[T0 + g]) : "memory");
+    for (int c = 0; c < C; ++c) asm volatile("" ::"v"(dl[c]), "v"(vl[c]) : "memory");
+#pragma unroll
+    for (int i = 0; i < PL::size(g); ++i)
+        asm volatile("" ::"v"(dbot[PL::start(g) + i]), "v"(vbot[PL::start(g) + i]) : "memory");
 }
 
-// Rows T0 .. T0 + G - 1 of one chain lane at one step.  Per row, with dl / vl
-// the values from the row below and dbot / vbot the row's own values from the
-// previous column:
+// Group g's rows of one chain lane for the C columns of one step, column by
+// column (a row's state after column c is its input at column c + 1).  Per
+// row, with dl / vl the values from the row below and dbot / vbot the row's
+// own values from the previous column:
 //   DFD   (.cu:458-641)  top = max(d, min(dl, dbot)),  dl' = min(top, dbot)
 //   VE-FD (.cu:200-432)  in = min(vl, vbot),  T = max(wt, in),  vl' = R = max(wr, in)
-template <int R, int T0, int G>
-__device__ inline void chain_rows(const double (&wv)[G][3], double (&dbot)[R], double (&vbot)[R], double &dl,
-                                  double &vl) {
+template <int R, int C, int g>
+__device__ inline void chain_rows(const double (&wv)[C][ChainPlan<R, C>::GMAX][3], double (&dbot)[R],
+                                  double (&vbot)[R], double (&dl)[C], double (&vl)[C]) {
+    using PL = ChainPlan<R, C>;
+    // One wait for the whole group (its loads were issued a group earlier),
+    // not one counter wait per load.
 #pragma unroll
-    for (int g = 0; g < G; ++g) {
-        const int t = T0 + g;
-        const double d = wv[g][0], wr = wv[g][1], wt = wv[g][2];
-        // dl' = min(top, dbot) = min(max(d, dl), dbot) (min distributes over
-        // max, and these doubles are never NaN): two operations on the serial
-        // path instead of three, and no copy of the old dbot.
-        const double dn = dmin(dmax(d, dl), dbot[t]);
-        dbot[t] = dmax(d, dmin(dl, dbot[t]));
-        dl = dn;
-        const double vin = dmin(vl, vbot[t]);
-        vbot[t] = dmax(wt, vin);
-        vl = dmax(wr, vin);
-    }
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int i = 0; i < PL::size(g); ++i)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) asm volatile("" ::"v"(wv[c][i][k]));
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int i = 0; i < PL::size(g); ++i) {
+            const int t = PL::start(g) + i;
+            const double d = wv[c][i][0], wr = wv[c][i][1], wt = wv[c][i][2];
+            // dl' = min(top, dbot) = min(max(d, dl), dbot) (min distributes
+            // over max, and these doubles are never NaN): two operations on
+            // the serial path instead of three, and no copy of the old dbot.
+            const double dn = dmin(dmax(d, dl[c]), dbot[t]);
+            dbot[t] = dmax(d, dmin(dl[c], dbot[t]));
+            dl[c] = dn;
+            const double vin = dmin(vl[c], vbot[t]);
+            vbot[t] = dmax(wt, vin);
+            vl[c] = dmax(wr, vin);
+        }
 }
 
 // Columns of the band-boundary rings in LDS (incoming and outgoing).
@@ -210,56 +243,80 @@ struct BoundsSync {
     int *out_done;   // outgoing columns the courier has published
 };
 
-template <int R, int KS>
-__device__ inline void bounds_chain(const BoundsArgs &a, const int band, const double *wring,
-                                    const double *inring, double *outring, const BoundsSync &sy,
-                                    const int npw) {
+// Calls f(integral_constant<int, g>) for g = G .. ChainPlan<R, C>::NG - 1.
+template <int R, int C, int G, typename F>
+__device__ inline void for_groups(F &&f) {
+    if constexpr (G < ChainPlan<R, C>::NG) {
+        f(std::integral_constant<int, G>{});
+        for_groups<R, C, G + 1>(f);
+    }
+}
+
+template <int R, int KS, int C>
+__device__ inline void bounds_chain(const BoundsArgs &a, const int band, const double *wring, double *inring,
+                                    double *outring, const BoundsSync &sy, const int npw) {
+    using PL = ChainPlan<R, C>;
+    constexpr int NG = PL::NG;
+    static_assert(NG >= 2, "the pipeline needs two row groups");
     const int lane = (int)(threadIdx.x & 63);
     const double big = (double)FLT_MAX;
-    // Steps and columns in 32 bits (the launch requires n < 2^31 - 64), so
-    // that the loop's bookkeeping stays on the scalar unit.
+    // Steps and columns in 32 bits (the launch requires n < 2^31 - 64 and, at
+    // two columns per step, n even), so that the loop's bookkeeping stays on
+    // the scalar unit.  Lane l evaluates columns C(s - l) .. C(s - l) + C - 1
+    // at step s.
     const int cols = (int)a.cols;
-    const int nsteps = cols + 63;
+    const int nsteps = cols / C + 63;
     double dbot[R], vbot[R];
 #pragma unroll
     for (int t = 0; t < R; ++t) {
         dbot[t] = big;
         vbot[t] = -big;  // column 0's weights then set the row's VE-FD entry value
     }
-    double dout = big, vout = big;
+    double dout[C], vout[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) dout[c] = vout[c] = big;
     const bool has_pred = band > 0;
     const bool has_succ = band + 1 < a.nbands;
     // Terminal cell (m-1, n-1): when this band holds row m - 1, it is row tl
-    // of lane tlane, which reaches column n - 1 at step n - 1 + tlane.
+    // of lane tlane, which reaches column n - 1 (the last column of a step) at
+    // step n / C - 1 + tlane.
     const long long toff = a.rows - 1 - (long long)band * 64 * R;
     const bool has_term = toff >= 0 && toff < 64LL * R;
     const int tlane = has_term ? (int)(toff / R) : -1;
     const int tl = has_term ? (int)(toff - (long long)tlane * R) : 0;
-    const int s_term = has_term ? cols - 1 + tlane : -1;
+    const int s_term = has_term ? cols / C - 1 + tlane : -1;
     int avail = 0, in_known = 0, done_known = 0;
     int slot = 0;
     if (a.tlog && lane == 0) a.tlog[4 * band] = wall_clock64();
+    // The first band's row -1 (lane 0's input from below) holds DFD's 0 at
+    // column 0 and +big after it, VE-FD's +big: two entries of its own ring.
+    if (!has_pred && lane == 0) {
+        inring[0] = 0.0;
+        inring[1] = big;
+        inring[kRing] = big;
+        inring[kRing + 1] = big;
+    }
     // The chain is the band's serial path and every later band waits on it:
     // its SIMD issues for it first, producers fill the remaining cycles.
     __builtin_amdgcn_s_setprio(3);
 
-    // Software pipeline over three row groups A, B, C: a group's LDS loads
-    // are issued while the group before it computes (B's and C's during the
-    // same step, A's and the input from below during C of the step before),
-    // so no LDS round trip sits between two steps' min/max chains, and at most
-    // two groups' weights occupy registers at a time.  Loads are issued on
-    // every path (the last step reads a slot and a ring entry nobody uses), so
-    // that the compiler's LDS counter waits stay exact.  Polls, which wait for
-    // all of this wave's LDS operations, come right after a group's compute.
-    constexpr int GA = (R + 2) / 3, GB = (R - GA + 1) / 2, GC = R - GA - GB;
-    double wa[GA][3], wb[GB > 0 ? GB : 1][3], wc[GC > 0 ? GC : 1][3];
-    double bD, bV;  // the courier's ring entries for the next step's input from below
+    // Software pipeline over the row groups: a group's LDS loads are issued
+    // while the group before it computes (group 0's, with the input from
+    // below, while the last group of the step before computes), so no LDS
+    // round trip sits between two steps' min/max chains, and two groups'
+    // weights occupy registers at a time.  Loads are issued on every path
+    // (the last step reads a slot and ring entries nobody uses), so that the
+    // compiler's LDS counter waits stay exact.  Polls, which wait for all of
+    // this wave's LDS operations, come right after a group's compute.
+    double w[NG][C][PL::GMAX][3];
+    double bD[C], bV[C];  // the next step's input from below (lane 0)
     auto wait_pred = [&](const int st) __attribute__((always_inline)) {
-        // Row 64Rb - 1 at column st (lane 0's input from below).
-        if (has_pred && st < cols && st >= in_known) {
+        // Row 64Rb - 1 at columns C st .. C st + C - 1.
+        const int need = (C * st + C < cols) ? C * st + C : cols;
+        if (has_pred && need > in_known) {
             for (;;) {
                 in_known = __builtin_amdgcn_readfirstlane(lds_get(sy.in_avail));
-                if (st < in_known) break;
+                if (need <= in_known) break;
                 __builtin_amdgcn_s_sleep(1);
             }
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -270,8 +327,8 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, const d
         if (st >= avail) {
             for (;;) {
                 int mn = lds_get(&sy.prod[0]);
-                for (int w = 1; w < npw; ++w) {
-                    const int v = lds_get(&sy.prod[w]);
+                for (int k = 1; k < npw; ++k) {
+                    const int v = lds_get(&sy.prod[k]);
                     mn = v < mn ? v : mn;
                 }
                 avail = __builtin_amdgcn_readfirstlane(mn);
@@ -282,61 +339,62 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, const d
         }
     };
     auto load_in = [&](const int st) __attribute__((always_inline)) {
-        const int r = (st < cols ? st : cols - 1) & (kRing - 1);
-        bD = inring[r];
-        bV = inring[kRing + r];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            int col = C * st + c;
+            col = col < cols ? col : cols - 1;
+            const int r = has_pred ? (col & (kRing - 1)) : (col < 1 ? col : 1);
+            bD[c] = inring[r];
+            bV[c] = inring[kRing + r];
+        }
     };
     wait_pred(0);
     wait_prod(0);
     load_in(0);
-    chain_load<0, GA>(wring + lane, wa);
+    chain_load<R, C, 0>(wring + lane, w[0]);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 
     // Every lane runs every step: before its first column the producers'
     // weights keep its rows at their initial values, and after its last
     // column it computes values nobody reads (the terminal cell is written at
-    // its own step, and lane 63's hand-off column never passes n - 1).
+    // its own step, and lane 63's hand-off columns never pass n - 1).
     for (int s = 0; s < nsteps; ++s) {
         const int nslot = (slot + 1 == KS) ? 0 : slot + 1;
-        const double *wcur = wring + slot * (R * 3 * 64) + lane;
-        const double *wnext = wring + nslot * (R * 3 * 64) + lane;
-        if constexpr (GB > 0) chain_load<GA, GB>(wcur, wb);
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        // Input from below: lane l-1's outputs of the step before, and for
-        // lane 0 the row under the band (the first band's row -1 holds DFD's
-        // 0 at column 0 and +big after it, VE-FD's +big).
-        double dl = shift_up(dout), vl = shift_up(vout);
-        const double inD = has_pred ? bD : (s == 0 ? 0.0 : big);
-        const double inV = has_pred ? bV : big;
-        if (lane == 0) {
-            dl = inD;
-            vl = inV;
+        const double *wcur = wring + slot * (C * R * 3 * 64) + lane;
+        const double *wnext = wring + nslot * (C * R * 3 * 64) + lane;
+        // Input from below: lane l-1's outputs of the step before (same
+        // columns), and for lane 0 the row under the band.
+        double dl[C], vl[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            dl[c] = dpp_f64(bD[c], dout[c], 0);
+            vl[c] = dpp_f64(bV[c], vout[c], 0);
         }
-        chain_rows<R, 0, GA>(wa, dbot, vbot, dl, vl);
-        chain_done<R, 0, GA>(dbot, vbot, dl, vl);
-        if constexpr (GC > 0) chain_load<GA + GB, GC>(wcur, wc);
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if constexpr (GB > 0) {
-            chain_rows<R, GA, GB>(wb, dbot, vbot, dl, vl);
-            chain_done<R, GA, GB>(dbot, vbot, dl, vl);
+        for_groups<R, C, 0>([&](auto G) __attribute__((always_inline)) {
+            constexpr int g = decltype(G)::value;
+            if constexpr (g + 1 < NG) {
+                chain_load<R, C, g + 1>(wcur, w[g + 1]);
+            } else {
+                if (s + 1 < nsteps) {
+                    wait_prod(s + 1);
+                    wait_pred(s + 1);
+                }
+                chain_load<R, C, 0>(wnext, w[0]);
+                load_in(s + 1);
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            chain_rows<R, C, g>(w[g], dbot, vbot, dl, vl);
+            chain_done<R, C, g>(dbot, vbot, dl, vl);
+        });
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            dout[c] = dl[c];
+            vout[c] = vl[c];
         }
-        if (s + 1 < nsteps) {
-            wait_prod(s + 1);
-            wait_pred(s + 1);
-        }
-        chain_load<0, GA>(wnext, wa);
-        load_in(s + 1);
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if constexpr (GC > 0) {
-            chain_rows<R, GA + GB, GC>(wc, dbot, vbot, dl, vl);
-            chain_done<R, GA + GB, GC>(dbot, vbot, dl, vl);
-        }
-        dout = dl;
-        vout = vl;
         if (s == s_term) {
             // Terminal cell: its DFD value is the row's top; its VE-FD value
             // is min(T, R), where R left row tl and, rows above tl being
-            // padding that passes VE-FD values through, is vout.
+            // padding that passes VE-FD values through, is the lane's output.
             if (lane == tlane) {
                 double topD = dbot[0], topV = vbot[0];
 #pragma unroll
@@ -344,7 +402,7 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, const d
                     topD = (tl == t) ? dbot[t] : topD;
                     topV = (tl == t) ? vbot[t] : topV;
                 }
-                const double resV = dmin(topV, vout);
+                const double resV = dmin(topV, vout[C - 1]);
                 a.result[0] = __builtin_sqrt(topD);
                 a.result[1] = topD;
                 a.result[2] = __builtin_sqrt(resV);
@@ -355,24 +413,27 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, const d
         if (lane == 0) lds_put(sy.cons, s + 1);
         slot = nslot;
 
-        // Row 64R(b+1) - 1 at column s - 63 (lane 63's output) into the
-        // outgoing ring, for the courier to publish.
+        // Row 64R(b+1) - 1 at columns C(s - 63) .. (lane 63's outputs) into
+        // the outgoing ring, for the courier to publish.
         if (has_succ && s >= 63) {
-            const int col = s - 63;
-            if (col >= done_known + kRing) {
+            const int col = C * (s - 63);
+            if (col + C > done_known + kRing) {
                 for (;;) {
                     done_known = __builtin_amdgcn_readfirstlane(lds_get(sy.out_done));
-                    if (col < done_known + kRing) break;
+                    if (col + C <= done_known + kRing) break;
                     __builtin_amdgcn_s_sleep(1);
                 }
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
             }
             if (lane == 63) {
-                const int r = col & (kRing - 1);
-                outring[r] = dout;
-                outring[kRing + r] = vout;
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const int r = (col + c) & (kRing - 1);
+                    outring[r] = dout[c];
+                    outring[kRing + r] = vout[c];
+                }
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                lds_put(sy.out_avail, col + 1);
+                lds_put(sy.out_avail, col + C);
             }
         }
     }
@@ -389,7 +450,7 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, const d
 // publishes the chain's outgoing columns (scoped stores, s_waitcnt, flag), as
 // many columns per round as are ready, up to 64.  NK values per column
 // (bound sweep: DFD and VE-FD; decision: one); slots are [buffer][NK][cols].
-template <int NK = 2>
+template <int NK = 2, int C = 1>
 __device__ inline void bounds_courier(const BoundsArgs &a, const int band, double *inring,
                                       const double *outring, const BoundsSync &sy) {
     const int lane = (int)(threadIdx.x & 63);
@@ -403,8 +464,10 @@ __device__ inline void bounds_courier(const BoundsArgs &a, const int band, doubl
     while (in_next < cols || out_next < cols) {
         bool moved = false;
         if (in_next < cols) {
-            // Ring slots free: the chain has consumed columns below `cons`.
-            const long long room = (long long)__builtin_amdgcn_readfirstlane(lds_get(sy.cons)) + kRing - in_next;
+            // Ring slots free: the chain has consumed the columns of the
+            // steps below `cons` (C per step).
+            const long long room =
+                (long long)C * __builtin_amdgcn_readfirstlane(lds_get(sy.cons)) + kRing - in_next;
             if (room > 0) {
                 const long long f = __builtin_amdgcn_readfirstlane(flag_load<SYNC_SCOPED>(&a.progress[band - 1]));
                 long long n = f - in_next;
@@ -453,18 +516,21 @@ __device__ inline void bounds_courier(const BoundsArgs &a, const int band, doubl
     }
 }
 
-// Columns of Q staged in LDS for the producers: four 64-column chunks,
-// component-major (component k of column j at k * kQRing + (j & (kQRing - 1))).
-constexpr int kQRing = 256;
+// Columns of Q staged in LDS for the producers: 4C chunks of 64 columns,
+// component-major (component k of column j at k * qring_cols(C) + (j mod
+// qring_cols(C))).
+constexpr int qring_cols(int C) { return 256 * C; }
 
-template <int D, int R, int NPW, int KS>
+template <int D, int R, int NPW, int KS, int C>
 __device__ inline void bounds_producer(const BoundsArgs &a, const int band, const int pw, double *wring,
                                        double *qring, int *prod, int *cons) {
+    constexpr int QR = qring_cols(C);
     // Rows t = pw, pw + NPW, ... (< R) of every lane.
     constexpr int RP = (R + NPW - 1) / NPW;
     const int nmine = (R - pw + NPW - 1) / NPW;
     const int lane = (int)(threadIdx.x & 63);
-    const long long rows = a.rows, cols = a.cols;
+    const long long rows = a.rows;
+    const int cols = (int)a.cols;
     const double big = (double)FLT_MAX;
     const double *P = a.P, *Q = a.Q;
     const long long i0 = (long long)band * 64 * R + (long long)R * lane;
@@ -486,33 +552,34 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         }
         ab2[k] = acc;
     }
-    // Q ring.  Every producer wave stores each chunk itself (the same values),
-    // chunk c + 1 at its step 64c, from registers loaded one chunk earlier.
-    // A wave at step s reads columns s - 63 .. s; producers stay within KS
-    // (< 64) steps of each other (the chain waits for the slowest, and each
-    // runs at most KS steps ahead of the chain), so no wave still reads chunk
-    // c - 3, whose slots chunk c + 1 takes, and none reads chunk c + 1 before
+    // Q ring.  Lane l reads columns C(s - l) .. C(s - l) + C - 1 at step s.
+    // Every producer wave stores each chunk itself (the same values): the
+    // chunk after the one holding column Cs, whenever Cs is a multiple of 64,
+    // from registers loaded one chunk earlier.  Producers stay within KS steps
+    // of each other (the chain waits for the slowest, and each runs at most KS
+    // steps ahead of the chain), so no wave still reads the chunk 4C chunks
+    // back, whose slots the stored chunk takes, and none reads a chunk before
     // its own store of it (a wave's LDS operations complete in order).
     double qn[D];
     {
-        const long long c0 = lane < cols ? lane : cols - 1;
-        const long long c1 = 64 + lane < cols ? 64 + lane : cols - 1;
+        const int c0 = lane < cols ? lane : cols - 1;
+        const int c1 = 64 + lane < cols ? 64 + lane : cols - 1;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            qring[k * kQRing + lane] = Q[c0 * D + k];
-            qn[k] = Q[c1 * D + k];
+            qring[k * QR + lane] = Q[(long long)c0 * D + k];
+            qn[k] = Q[(long long)c1 * D + k];
         }
     }
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     const double r2 = a.r2;
-    const long long nsteps = cols + 63;
-    long long cons_seen = 0;
+    const int nsteps = cols / C + 63;
+    int cons_seen = 0;
     int slot = 0;
-    // Lane l at step s evaluates column j = s - l.  Carried from the step
-    // before: q_{j-1}, and p_i - q_{j-1} of each row, which is the right
-    // weight's p_i - a term (the same operands and operation as the DFD
-    // difference one step earlier, so the same value).  Values at j <= 0 feed
-    // only weights that the boundary rule below overwrites.
+    // Carried from the column before: q_{j-1}, and p_i - q_{j-1} of each row,
+    // which is the right weight's p_i - a term (the same operands and
+    // operation as the DFD difference one column earlier, so the same value).
+    // Values at j <= 0 feed only weights that the boundary rules below
+    // overwrite.
     double qA[D], qB[D], xA[RP][D], xB[RP][D];
 #pragma unroll
     for (int c = 0; c < D; ++c) qA[c] = 0.0;
@@ -521,22 +588,12 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
 #pragma unroll
         for (int c = 0; c < D; ++c) xA[k][c] = 0.0;
 
-    auto step = [&](const long long s, const double(&qa)[D], double(&qb)[D], const double(&xp)[RP][D],
-                    double(&xn)[RP][D]) __attribute__((always_inline)) {
-        if ((s & 63) == 0) {
-            const long long c = s + 64 + lane;  // chunk s / 64 + 1
-            long long cn = c + 64;              // chunk s / 64 + 2, stored 64 steps later
-            cn = cn < cols ? cn : cols - 1;
+    // Weights of column j for this wave's rows into w (one column of a slot).
+    auto column = [&](const int j, const double(&qa)[D], double(&qb)[D], const double(&xp)[RP][D],
+                      double(&xn)[RP][D], double *w, const bool edge) __attribute__((always_inline)) {
+        const int qi = j & (QR - 1);
 #pragma unroll
-            for (int k = 0; k < D; ++k) {
-                qring[k * kQRing + (int)(c & (kQRing - 1))] = qn[k];
-                qn[k] = Q[cn * D + k];
-            }
-        }
-        const long long j = s - lane;
-        const int qi = (int)(j & (kQRing - 1));
-#pragma unroll
-        for (int k = 0; k < D; ++k) qb[k] = qring[k * kQRing + qi];
+        for (int k = 0; k < D; ++k) qb[k] = qring[k * QR + qi];
         // Column terms of segment [q_{j-1}, q_j].
         double cab[D], cab2 = 0.0;
 #pragma unroll
@@ -544,16 +601,6 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
             cab[k] = qb[k] - qa[k];
             cab2 = __builtin_fma(cab[k], cab[k], cab2);
         }
-        if (s >= cons_seen + KS) {
-            for (;;) {
-                cons_seen = __builtin_amdgcn_readfirstlane(lds_get(cons));
-                if (s < cons_seen + KS) break;
-                __builtin_amdgcn_s_sleep(1);
-            }
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        }
-        double *w = wring + (long long)slot * (R * 3 * 64) + lane;
-        const bool edge = s < 64 || band == 0 || band + 1 == a.nbands;
 #pragma unroll
         for (int k = 0; k < RP; ++k) {
             if (RP * NPW != R && k == RP - 1 && k >= nmine) break;
@@ -595,27 +642,71 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
                 }
             }
         }
+    };
+    auto step_begin = [&](const int s) __attribute__((always_inline)) {
+        if (((C * s) & 63) == 0) {
+            const int c = C * s + 64 + lane;  // the chunk after the one holding column Cs
+            int cn = c + 64;                  // and the chunk after that, stored 64 columns later
+            cn = cn < cols ? cn : cols - 1;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                qring[k * QR + (c & (QR - 1))] = qn[k];
+                qn[k] = Q[(long long)cn * D + k];
+            }
+        }
+        if (s >= cons_seen + KS) {
+            for (;;) {
+                cons_seen = __builtin_amdgcn_readfirstlane(lds_get(cons));
+                if (s < cons_seen + KS) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        }
+    };
+    auto step_end = [&](const int s) __attribute__((always_inline)) {
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this step's weights are in LDS
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if (lane == 0) lds_put(&prod[pw], (int)(s + 1));
+        if (lane == 0) lds_put(&prod[pw], s + 1);
         slot = (slot + 1 == KS) ? 0 : slot + 1;
     };
-    // Two steps per trip, the carried values alternating between the A and B
-    // registers instead of being copied.
-    long long s = 0;
-    for (; s + 1 < nsteps; s += 2) {
-        step(s, qA, qB, xA, xB);
-        step(s + 1, qB, qA, xB, xA);
+    const bool edge_band = band == 0 || band + 1 == a.nbands;
+    if constexpr (C == 2) {
+        // The carried values alternate between the A and B registers from
+        // one column to the next instead of being copied.
+        for (int s = 0; s < nsteps; ++s) {
+            step_begin(s);
+            double *w = wring + slot * (C * R * 3 * 64) + lane;
+            const bool edge = s < 64 || edge_band;
+            const int j = C * (s - lane);
+            column(j, qA, qB, xA, xB, w, edge);
+            column(j + 1, qB, qA, xB, xA, w + R * 3 * 64, edge);
+            step_end(s);
+        }
+    } else {
+        // Two steps per trip, the carried values alternating between the A
+        // and B registers instead of being copied.
+        auto one = [&](const int s, const double(&qa)[D], double(&qb)[D], const double(&xp)[RP][D],
+                       double(&xn)[RP][D]) __attribute__((always_inline)) {
+            step_begin(s);
+            double *w = wring + slot * (R * 3 * 64) + lane;
+            column(s - lane, qa, qb, xp, xn, w, s < 64 || edge_band);
+            step_end(s);
+        };
+        int s = 0;
+        for (; s + 1 < nsteps; s += 2) {
+            one(s, qA, qB, xA, xB);
+            one(s + 1, qB, qA, xB, xA);
+        }
+        if (s < nsteps) one(s, qA, qB, xA, xB);
     }
-    if (s < nsteps) step(s, qA, qB, xA, xB);
 }
 
 // LDS of one workgroup of the fused sweep.
-template <int R, int NPW> struct BoundsShared {
-    static constexpr int KS = BoundsRing<R, NPW>::KS;
-    double wring[KS * R * 3 * 64];
-    double qring[3 * kQRing];
+template <int R, int NPW, int C = 1> struct BoundsShared {
+    static constexpr int KS = BoundsRing<R, NPW, C>::KS;
+    double wring[KS * C * R * 3 * 64];
+    double qring[3 * qring_cols(C)];
     double inring[2 * kRing];
     double outring[2 * kRing];
     int sprod[NPW];
@@ -625,9 +716,10 @@ template <int R, int NPW> struct BoundsShared {
 
 // One band of one pair, by every wave of the workgroup.  The caller has
 // cleared the sync words and ends with a barrier before reusing the LDS.
-template <int D, int R, int NPW>
-__device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsShared<R, NPW> &sh) {
-    constexpr int KS = BoundsRing<R, NPW>::KS;
+// C = 2 (two columns per step) requires n even.
+template <int D, int R, int NPW, int C = 1>
+__device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C> &sh) {
+    constexpr int KS = BoundsRing<R, NPW, C>::KS;
     const int wave = (int)(threadIdx.x >> 6);
     const BoundsSync sy{sh.sprod, &sh.swords[0], &sh.swords[1], &sh.swords[2], &sh.swords[3]};
     // Waves of a workgroup are dealt to the CU's four SIMDs in turn, so wave 4
@@ -635,24 +727,25 @@ __device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsSh
     // the chain's SIMD keeps its issue slots for the chain.
     constexpr int cw = (NPW + 2 > 4) ? 4 : NPW + 1;
     if (wave == 0)
-        bounds_chain<R, KS>(a, band, sh.wring, sh.inring, sh.outring, sy, NPW);
+        bounds_chain<R, KS, C>(a, band, sh.wring, sh.inring, sh.outring, sy, NPW);
     else if (wave == cw)
-        bounds_courier<2>(a, band, sh.inring, sh.outring, sy);
+        bounds_courier<2, C>(a, band, sh.inring, sh.outring, sy);
     else
-        bounds_producer<D, R, NPW, KS>(a, band, wave - 1 - (wave > cw ? 1 : 0), sh.wring, sh.qring, sh.sprod,
-                                       sy.cons);
+        bounds_producer<D, R, NPW, KS, C>(a, band, wave - 1 - (wave > cw ? 1 : 0), sh.wring, sh.qring, sh.sprod,
+                                          sy.cons);
 }
 
-template <int R, int NPW>
-__device__ inline void bounds_clear(BoundsShared<R, NPW> &sh) {
+template <int R, int NPW, int C>
+__device__ inline void bounds_clear(BoundsShared<R, NPW, C> &sh) {
     for (int w = 0; w < NPW; ++w) sh.sprod[w] = 0;
     for (int k = 0; k < 4; ++k) sh.swords[k] = 0;
 }
 
-template <int D, int R, int NPW>
+template <int D, int R, int NPW, int C>
 __global__ __launch_bounds__(64 * (NPW + 2)) void bounds_sweep(BoundsArgs a) {
     static_assert(NPW >= 1 && NPW <= R, "one to R producer waves");
-    __shared__ BoundsShared<R, NPW> sh;
+    static_assert(C == 1 || C == 2, "one or two columns per step");
+    __shared__ BoundsShared<R, NPW, C> sh;
     for (;;) {
         if (threadIdx.x == 0) {
             sh.item = atomicAdd(a.queue, 1);
@@ -661,7 +754,7 @@ __global__ __launch_bounds__(64 * (NPW + 2)) void bounds_sweep(BoundsArgs a) {
         __syncthreads();
         const int band = sh.item;
         if (band >= a.nbands) break;
-        bounds_band<D, R, NPW>(a, band, sh);
+        bounds_band<D, R, NPW, C>(a, band, sh);
         __syncthreads();
     }
 }

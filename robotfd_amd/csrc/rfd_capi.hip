@@ -215,10 +215,11 @@ rfd_status run_sweep(rfd_session *s, const T *P, const T *Q, const T *qseg, doub
 }
 
 // Fused DFD + VE-FD sweep.  Configurations: rows per lane R and producer
-// waves NPW (one chain wave besides), encoded as 10 * R + NPW.
-template <int D, int R, int NPW>
+// waves NPW (one chain wave besides), encoded as 10 * R + NPW (100 * R + NPW
+// from 10 producer waves on), plus 20000 for two columns per step.
+template <int D, int R, int NPW, int C = 1>
 hipError_t launch_bounds_cfg(rfd_ctx *ctx, int64_t nb, const rfd::BoundsArgs &a, int *grid_out) {
-    auto kern = rfd::bounds_sweep<D, R, NPW>;
+    auto kern = rfd::bounds_sweep<D, R, NPW, C>;
     // Optional cap on workgroups per CU (RFD_WGCU): unused dynamic LDS sized so
     // that one more workgroup does not fit.  Bands on a CU shared by more
     // workgroups run slower, and every later band waits on the slowest.
@@ -242,7 +243,8 @@ hipError_t launch_bounds_cfg(rfd_ctx *ctx, int64_t nb, const rfd::BoundsArgs &a,
 
 bool valid_bounds_cfg(int cfg) {
     return cfg == 42 || cfg == 62 || cfg == 63 || cfg == 72 || cfg == 73 || cfg == 83 || cfg == 84 || cfg == 21 ||
-           cfg == 136 || cfg == 137 || cfg == 77 || cfg == 107 || cfg == 1310 || cfg == 1313 || cfg == 1012;
+           cfg == 136 || cfg == 137 || cfg == 77 || cfg == 107 || cfg == 1310 || cfg == 1313 || cfg == 1012 ||
+           cfg == 20077 || cfg == 20107 || cfg == 20137 || cfg == 21310;
 }
 
 template <int D>
@@ -262,6 +264,10 @@ hipError_t launch_bounds(rfd_ctx *ctx, int cfg, int64_t nb, const rfd::BoundsArg
         case 1310: return launch_bounds_cfg<D, 13, 10>(ctx, nb, a, grid);
         case 1313: return launch_bounds_cfg<D, 13, 13>(ctx, nb, a, grid);
         case 1012: return launch_bounds_cfg<D, 10, 10>(ctx, nb, a, grid);
+        case 20077: return launch_bounds_cfg<D, 7, 7, 2>(ctx, nb, a, grid);
+        case 20107: return launch_bounds_cfg<D, 10, 7, 2>(ctx, nb, a, grid);
+        case 20137: return launch_bounds_cfg<D, 13, 7, 2>(ctx, nb, a, grid);
+        case 21310: return launch_bounds_cfg<D, 13, 10, 2>(ctx, nb, a, grid);
         default: return launch_bounds_cfg<D, 4, 2>(ctx, nb, a, grid);
     }
 }
@@ -274,13 +280,21 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
     // waves, 10 at 13 rows per lane), or 4 rows per lane with 2 producer waves
     // (several bands per CU) for pairs too short to give every CU a band of 7
     // rows per lane.
+    // Two columns per step (20000 + the one-column code, opt-in) need even n;
+    // for odd n the one-column kernel runs.
+    if (s->n >= (int64_t)0x7fffffff - 64) {
+        set_err(ctx, "the fused bound sweep needs n < 2^31 - 64 (n=%lld)", (long long)s->n);
+        return RFD_ERR_ARG;
+    }
     int cfg = ctx->bounds_cfg;
     if (cfg == 0) {
         const int64_t per = 64 * (int64_t)ctx->cu_count;
         const int64_t need = (s->m + per - 1) / per;  // rows per lane for one band per CU
         cfg = need <= 4 ? 42 : need <= 7 ? 77 : need <= 10 ? 107 : 1310;
     }
-    const int R = cfg >= 1000 ? cfg / 100 : cfg / 10;  // 10 R + NPW, or 100 R + NPW for NPW >= 10
+    if (cfg >= 20000 && (s->n & 1)) cfg -= 20000;
+    const int base = cfg % 10000;
+    const int R = base >= 1000 ? base / 100 : base / 10;  // 10 R + NPW, or 100 R + NPW for NPW >= 10
     const int64_t rows = s->m, cols = s->n;
     const int64_t nb = (rows + 64 * R - 1) / (64 * R);
     rfd::BoundsArgs a{};

@@ -61,7 +61,10 @@ def check_pair(ctx, P, Q):
     assert ve == O.vefd(P, Q, lb)
 
 
-@pytest.mark.parametrize("cfg", [21, 42, 62, 63, 72, 73, 77, 83, 84, 107, 136, 137, 1012, 1310, 1313])
+# 20000 + c: the same configuration at two columns per step (even n; odd n
+# runs the one-column kernel).
+@pytest.mark.parametrize("cfg", [21, 42, 62, 63, 72, 73, 77, 83, 84, 107, 136, 137, 1012, 1310, 1313,
+                                 20077, 20107, 20137, 21310])
 @pytest.mark.parametrize("dims", [2, 3])
 def test_fused_configs_bit_exact(contexts, cfg, dims):
     ctx = contexts(RFD_BOUNDS=cfg)
@@ -75,12 +78,25 @@ def test_fused_configs_bit_exact(contexts, cfg, dims):
     assert st["bounds_f64"]["launches"] == len(SHAPES) and "dfd" not in st
 
 
-@pytest.mark.parametrize("cfg", [42, 84, 136, 137])
+@pytest.mark.parametrize("cfg", [42, 84, 136, 137, 20137, 21310])
 def test_fused_workgroups_take_several_bands(contexts, cfg):
     ctx = contexts(RFD_BOUNDS=cfg, RFD_MAX_WAVES=3)
     rng = np.random.default_rng(5)
-    P, Q = walk(rng, 64 * 8 * 7 + 5, 3), walk(rng, 333, 3)
+    n = 334 if cfg >= 20000 else 333
+    P, Q = walk(rng, 64 * 8 * 7 + 5, 3), walk(rng, n, 3)
     say("wrap", cfg)
+    check_pair(ctx, P, Q)
+
+
+@pytest.mark.parametrize("m,n", [(2, 2), (64, 130), (833, 2), (832 * 3 + 1, 4096), (5000, 1000)])
+def test_two_column_sweep_even_shapes(contexts, m, n):
+    """Two columns per step: the terminal column is the second of its step,
+    the band fill is 63 steps of two columns, the hand-off rings carry two
+    columns per step."""
+    ctx = contexts(RFD_BOUNDS=21310)
+    rng = np.random.default_rng(m + 7 * n)
+    P, Q = walk(rng, m, 3), walk(rng, n, 3)
+    say("two-column", (m, n))
     check_pair(ctx, P, Q)
 
 
