@@ -19,7 +19,10 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "librfd.so")
 
-HIP_SOURCES = ["rfd_capi.hip", "rfd_batch.hip", "rfd_mlp.hip"]
+HIP_SOURCES = ["rfd_capi.hip", "rfd_batch.hip", "rfd_bounds.hip", "rfd_mlp.hip"]
+# Per-file additions: the fused bound sweep's unit has no NaN on its path and
+# is compiled without NaN semantics (see the head of rfd_bounds.hip).
+EXTRA_FLAGS = {"rfd_bounds.hip": ["-fno-honor-nans"]}
 CXX_SOURCES = ["wrapper.cpp"]
 
 HIPCC_FLAGS = [
@@ -51,7 +54,7 @@ def build(verbose: bool = False) -> str:
         if not os.path.exists(path):
             continue
         obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
-        cmd = [hipcc(), *HIPCC_FLAGS, "-c", path, "-o", obj]
+        cmd = [hipcc(), *HIPCC_FLAGS, *EXTRA_FLAGS.get(src, []), "-c", path, "-o", obj]
         if src.endswith(".cpp"):
             cmd = [hipcc(), "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
                    f"-I{INCLUDE}", f"-I{CSRC}", "-c", path, "-o", obj]

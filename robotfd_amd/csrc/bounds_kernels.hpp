@@ -142,20 +142,27 @@ __device__ inline double seg_w_ap(const double *x, const double *a, const double
     return w;
 }
 
-// min / max of doubles that are never NaN: a bare v_min_f64 / v_max_f64.
-// __builtin_fmin would first quiet each operand the compiler cannot prove
-// canonical (LDS loads, DPP results, loop-carried values), one extra
-// instruction per operand on the chain's serial path.
-__device__ inline double dmin(double a, double b) {
+// min / max of doubles that are never NaN.  Where the translation unit is
+// compiled without NaN semantics (rfd_bounds.hip) these are the compiler's
+// fmin / fmax, which then lower to bare v_min_f64 / v_max_f64 with known
+// latency.  Elsewhere they are inline asm: __builtin_fmin would first quiet
+// each operand the compiler cannot prove canonical (LDS loads, DPP results,
+// loop-carried values), one extra instruction per operand.
+#if defined(RFD_BOUNDS_NO_NANS)
+static __device__ inline double dmin(double a, double b) { return __builtin_fmin(a, b); }
+static __device__ inline double dmax(double a, double b) { return __builtin_fmax(a, b); }
+#else
+static __device__ inline double dmin(double a, double b) {
     double r;
     asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
     return r;
 }
-__device__ inline double dmax(double a, double b) {
+static __device__ inline double dmax(double a, double b) {
     double r;
     asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
     return r;
 }
+#endif
 
 // Row groups of the chain's software pipeline: NG groups of at most GMAX
 // rows, group g holding rows start(g) .. start(g + 1) - 1.  Three groups at

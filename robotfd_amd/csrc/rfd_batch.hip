@@ -97,28 +97,6 @@ void launch_batch_sweep(int rpl, int grid, hipStream_t st, const rfd::BatchArgs 
     else launch_batch_sweep_p<OP, false>(rpl, grid, st, a);
 }
 
-template <int R, int NPW>
-void launch_batch_bounds_cfg(rfd_ctx *ctx, hipStream_t st, const rfd::BatchArgs &a) {
-    auto kern = rfd::batch_bounds<R, NPW>;
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * (NPW + 2), 0) != hipSuccess || per_cu < 1)
-        per_cu = 1;
-    const int grid = std::max(1, std::min(ctx->cu_count * per_cu, ctx->max_waves));
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * (NPW + 2)), 0, st, a);
-}
-
-
-void launch_batch_bounds(rfd_ctx *ctx, int cfg, hipStream_t st, const rfd::BatchArgs &a) {
-    switch (cfg) {
-        case 21: launch_batch_bounds_cfg<2, 1>(ctx, st, a); break;
-        case 63: launch_batch_bounds_cfg<6, 3>(ctx, st, a); break;
-        case 137: launch_batch_bounds_cfg<13, 7>(ctx, st, a); break;
-        case 77: launch_batch_bounds_cfg<7, 7>(ctx, st, a); break;
-        case 1310: launch_batch_bounds_cfg<13, 10>(ctx, st, a); break;
-        default: launch_batch_bounds_cfg<4, 2>(ctx, st, a); break;
-    }
-}
-
 rfd::BatchArgs make_args(rfd_batch *b, int rpl, const rfd_opts &o, int prune) {
     rfd::BatchArgs a{};
     a.bounds_rows = b->ctx->batch_bounds >= 1000 ? b->ctx->batch_bounds / 100 : b->ctx->batch_bounds / 10;
@@ -163,7 +141,7 @@ void enqueue_rounds(rfd_batch *b, const rfd::BatchArgs &a, bool with_bounds, int
         if (ctx->batch_bounds) {
             // DFD and VE-FD of every pair in one fused round.
             hipLaunchKernelGGL(rfd::batch_plan, dim3(1), dim3(1024), 0, st, a, (int)rfd::PH_BOUNDS);
-            launch_batch_bounds(ctx, ctx->batch_bounds, st, a);
+            rfd_host::rfd_launch_batch_bounds(ctx, ctx->batch_bounds, st, a);
             hipLaunchKernelGGL(rfd::batch_update, dim3(pg), dim3(256), 0, st, a, (int)rfd::PH_BOUNDS);
         } else {
             const int phases[3] = {rfd::PH_DFD, rfd::PH_VEFD, rfd::PH_VEFD_REDO};

@@ -65,6 +65,9 @@ namespace {
 
 using rfd_host::set_err;
 using rfd_host::endpoint_bound;
+using rfd_host::rfd_launch_bounds;
+using rfd_host::rfd_valid_batch_bounds_cfg;
+using rfd_host::rfd_valid_bounds_cfg;
 
 int64_t nbands_for(int64_t rows) { return (rows + 63) / 64; }
 
@@ -214,64 +217,6 @@ rfd_status run_sweep(rfd_session *s, const T *P, const T *Q, const T *qseg, doub
     return RFD_OK;
 }
 
-// Fused DFD + VE-FD sweep.  Configurations: rows per lane R and producer
-// waves NPW (one chain wave besides), encoded as 10 * R + NPW (100 * R + NPW
-// from 10 producer waves on), plus 20000 for two columns per step.
-template <int D, int R, int NPW, int C = 1>
-hipError_t launch_bounds_cfg(rfd_ctx *ctx, int64_t nb, const rfd::BoundsArgs &a, int *grid_out) {
-    auto kern = rfd::bounds_sweep<D, R, NPW, C>;
-    // Optional cap on workgroups per CU (RFD_WGCU): unused dynamic LDS sized so
-    // that one more workgroup does not fit.  Bands on a CU shared by more
-    // workgroups run slower, and every later band waits on the slowest.
-    size_t dyn = 0;
-    if (ctx->wg_per_cu > 0 && ctx->lds_per_cu > 0) {
-        hipFuncAttributes fa;
-        hipError_t e0 = hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(kern));
-        if (e0 != hipSuccess) return e0;
-        const size_t want = (size_t)ctx->lds_per_cu / (size_t)(ctx->wg_per_cu + 1) + 1;
-        if (want > fa.sharedSizeBytes) dyn = want - fa.sharedSizeBytes;
-    }
-    int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * (NPW + 2), dyn);
-    if (e != hipSuccess) return e;
-    int64_t grid = std::min<int64_t>(nb, (int64_t)ctx->cu_count * std::max(per_cu, 1));
-    grid = std::max<int64_t>(1, std::min<int64_t>(grid, ctx->max_waves));  // RFD_MAX_WAVES caps workgroups too
-    *grid_out = (int)grid;
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * (NPW + 2)), dyn, ctx->stream, a);
-    return hipGetLastError();
-}
-
-bool valid_bounds_cfg(int cfg) {
-    return cfg == 42 || cfg == 62 || cfg == 63 || cfg == 72 || cfg == 73 || cfg == 83 || cfg == 84 || cfg == 21 ||
-           cfg == 136 || cfg == 137 || cfg == 77 || cfg == 107 || cfg == 1310 || cfg == 1313 || cfg == 1012 ||
-           cfg == 20077 || cfg == 20107 || cfg == 20137 || cfg == 21310;
-}
-
-template <int D>
-hipError_t launch_bounds(rfd_ctx *ctx, int cfg, int64_t nb, const rfd::BoundsArgs &a, int *grid) {
-    switch (cfg) {
-        case 21: return launch_bounds_cfg<D, 2, 1>(ctx, nb, a, grid);
-        case 62: return launch_bounds_cfg<D, 6, 2>(ctx, nb, a, grid);
-        case 63: return launch_bounds_cfg<D, 6, 3>(ctx, nb, a, grid);
-        case 72: return launch_bounds_cfg<D, 7, 2>(ctx, nb, a, grid);
-        case 73: return launch_bounds_cfg<D, 7, 3>(ctx, nb, a, grid);
-        case 83: return launch_bounds_cfg<D, 8, 3>(ctx, nb, a, grid);
-        case 84: return launch_bounds_cfg<D, 8, 4>(ctx, nb, a, grid);
-        case 136: return launch_bounds_cfg<D, 13, 6>(ctx, nb, a, grid);
-        case 137: return launch_bounds_cfg<D, 13, 7>(ctx, nb, a, grid);
-        case 77: return launch_bounds_cfg<D, 7, 7>(ctx, nb, a, grid);
-        case 107: return launch_bounds_cfg<D, 10, 7>(ctx, nb, a, grid);
-        case 1310: return launch_bounds_cfg<D, 13, 10>(ctx, nb, a, grid);
-        case 1313: return launch_bounds_cfg<D, 13, 13>(ctx, nb, a, grid);
-        case 1012: return launch_bounds_cfg<D, 10, 10>(ctx, nb, a, grid);
-        case 20077: return launch_bounds_cfg<D, 7, 7, 2>(ctx, nb, a, grid);
-        case 20107: return launch_bounds_cfg<D, 10, 7, 2>(ctx, nb, a, grid);
-        case 20137: return launch_bounds_cfg<D, 13, 7, 2>(ctx, nb, a, grid);
-        case 21310: return launch_bounds_cfg<D, 13, 10, 2>(ctx, nb, a, grid);
-        default: return launch_bounds_cfg<D, 4, 2>(ctx, nb, a, grid);
-    }
-}
-
 // One fused bound sweep of a session (m, n >= 2): DFD, DFD^2 and VE-FD.
 rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, double *vefd) {
     rfd_ctx *ctx = s->ctx;
@@ -321,7 +266,7 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
                        sentinel, s->evals);
     RFD_HIP(ctx, hipGetLastError());
     int grid = 0;
-    RFD_HIP(ctx, s->dims == 2 ? launch_bounds<2>(ctx, cfg, nb, a, &grid) : launch_bounds<3>(ctx, cfg, nb, a, &grid));
+    RFD_HIP(ctx, rfd_launch_bounds(ctx, s->dims, cfg, nb, a, &grid));
     RFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     double hres[4] = {0, 0, 0, 0};
     RFD_HIP(ctx, hipMemcpyAsync(hres, s->result, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -499,7 +444,7 @@ rfd_status rfd_create(rfd_ctx **out, int device) {
     if (const char *e = std::getenv("RFD_FUSED")) ctx->fused = std::atoi(e) != 0;
     if (const char *e = std::getenv("RFD_BOUNDS")) {
         int c = std::atoi(e);
-        if (valid_bounds_cfg(c)) ctx->bounds_cfg = c;
+        if (rfd_valid_bounds_cfg(c)) ctx->bounds_cfg = c;
     }
     if (const char *e = std::getenv("RFD_WGCU")) ctx->wg_per_cu = std::max(0, std::atoi(e));
     if (const char *e = std::getenv("RFD_DEC")) {
@@ -508,7 +453,7 @@ rfd_status rfd_create(rfd_ctx **out, int device) {
     }
     if (const char *e = std::getenv("RFD_BATCH_BOUNDS")) {
         const int c = std::atoi(e);
-        if (c == 0 || c == 21 || c == 42 || c == 63 || c == 77 || c == 137 || c == 1310) ctx->batch_bounds = c;
+        if (c == 0 || rfd_valid_batch_bounds_cfg(c)) ctx->batch_bounds = c;
     }
     {
         int lds = 0;

@@ -44,6 +44,23 @@ inline void set_err(rfd_ctx *ctx, const char *fmt, ...) {
 
 }  // namespace rfd_host
 
+namespace rfd {
+struct BoundsArgs;
+struct BatchArgs;
+}  // namespace rfd
+
+namespace rfd_host {
+
+// The fused bound sweep's launches (rfd_bounds.hip).  Configuration codes:
+// 10 * rows per lane + producer waves (100 * rows + waves from 10 waves on),
+// plus 20000 for two columns per step.
+bool rfd_valid_bounds_cfg(int cfg);
+hipError_t rfd_launch_bounds(rfd_ctx *ctx, int dims, int cfg, int64_t nb, const rfd::BoundsArgs &a, int *grid);
+bool rfd_valid_batch_bounds_cfg(int cfg);
+void rfd_launch_batch_bounds(rfd_ctx *ctx, int cfg, hipStream_t st, const rfd::BatchArgs &a);
+
+}  // namespace rfd_host
+
 #define RFD_HIP(ctx, call)                                                          \
     do {                                                                            \
         hipError_t e_ = (call);                                                     \
