@@ -8,8 +8,10 @@
 namespace Frechet {
 namespace Continuous {
 // Bracket [VE-FD, DFD], bisection while (ub - lb > eps), result (ub + lb) / 2.
-// Prints the reference's progress lines.  forceOversize is accepted and
-// ignored (the decision sweep is always the fused one).  Returns NaN on error.
+// Prints the reference's progress lines.  The decisions follow the
+// reference's path choice (.cu:87-96): reachable<D>'s interval form when
+// forceOversize is set or its interval arrays would not fit in free device
+// memory, else the two-phase interval form.  Returns NaN on error.
 double distance_cuda(const Curve &curve1, const Curve &curve2, const double eps,
                      bool forceOversize);
 }  // namespace Continuous

@@ -41,6 +41,21 @@ typedef enum {
                             FP64 decisions for the end game (certified bracket) */
 } rfd_precision;
 
+/* Interval form of the decision sweep.  The reference has two decision
+ * paths with the same reachability rules and slightly different interval
+ * arithmetic: reachable<D> (.cu:960-1256), and the two-phase
+ * interval_calculation<D> + reachability (.cu:674-958), which it runs when its
+ * four interval arrays fit in free device memory (.cu:87-96) unless
+ * forceOversize is set.  The two differ only in rounding near a critical
+ * radius and at zero-length segments (NaN intervals in the two-phase form). */
+typedef enum {
+    RFD_DECIDE_FUSED = 0,      /* reachable<D>'s intervals (default) */
+    RFD_DECIDE_TWO_PHASE = 1,  /* interval_calculation<D>'s intervals */
+    RFD_DECIDE_REFERENCE = 2   /* as the reference picks for each pair: two-phase when
+                                  4 * ((m-1)(n+m-4) + m) doubles plus CUs * 1024 bytes fit
+                                  in free device memory, else fused */
+} rfd_decide_form;
+
 typedef struct {
     int32_t stop_mode;     /* rfd_stop */
     int32_t precision;     /* rfd_precision */
@@ -53,6 +68,7 @@ typedef struct {
     int32_t prune;         /* VE-FD cells above DFD^2 skipped: -1 = library default, 0, 1 */
     int32_t spec_depth;    /* batch: levels of the bisection tree one decision round may
                               evaluate when few pairs remain (0 = library default, 1 = plain) */
+    int32_t decide_form;   /* rfd_decide_form of the decisions (default RFD_DECIDE_FUSED) */
 } rfd_opts;
 
 typedef struct {
@@ -117,6 +133,13 @@ rfd_status rfd_session_bounds(rfd_session *s, double *dfd, double *vefd, int run
  * be NULL) receives that rightUB.  fp32 != 0 evaluates the sweep in FP32. */
 rfd_status rfd_session_decide(rfd_session *s, double radius, int fp32, int *reachable,
                               double *terminal_ub);
+
+/* The decision path Cuda_intersection's constructor fixes for the session
+ * (.cu:87-96, 1433-1558): form is an rfd_decide_form; RFD_DECIDE_REFERENCE
+ * applies the reference's free-memory rule now, as its constructor does.
+ * *effective (may be NULL) receives the form in use (FUSED or TWO_PHASE).
+ * rfd_session_decide uses it; rfd_session_distance uses opts->decide_form. */
+rfd_status rfd_session_set_decide_form(rfd_session *s, int32_t form, int32_t *effective);
 
 /* Replaces Frechet::Continuous::distance_cuda (frechet.cpp:18-75) on a
  * session whose curves are already resident on the device. */

@@ -101,6 +101,41 @@ inline void free_iv(const T *c, const T *s0, const T *s1, int D, T r2, T A4, T i
     }
 }
 
+// The same interval as interval_calculation<D> forms it (.cu:722-782), the
+// decision's two-phase path for inputs whose interval arrays fit in device
+// memory (.cu:1450-1558).  It differs from free_iv in two places: the C term is
+// (sum of mi^2) - r2, the sum started at 0 (.cu:733-742, 765-770), where
+// reachable<D> starts it at -r2; and the clamps are comparisons (.cu:751-752,
+// 778-779), so a NaN end (0 * inf at a zero-length segment) stays NaN, where
+// reachable<D>'s max / min take it to 0 / 1.
+template <typename T>
+inline void free_iv_2ph(const T *c, const T *s0, const T *s1, int D, T r2, T A4, T inv2A,
+                        T &lb, T &ub) {
+    T B = T(0), mm = T(0);
+    for (int k = 0; k < D; ++k) {
+        T di = s1[k] - s0[k];
+        T mi = s0[k] - c[k];
+        B = fmaT(mi, di, B);
+        mm = fmaT(mi, mi, mm);
+    }
+    B = B * T(2);
+    const T C = mm - r2;
+    T disc = fmaT(B, B, -(A4 * C));
+    lb = T(-1);
+    ub = T(-2);
+    if (disc >= T(0)) {
+        T s = sqrtT(disc);
+        T ta = (-B - s) * inv2A;
+        T tb = (-B + s) * inv2A;
+        if (!(tb < T(0) || ta > T(1))) {
+            if (ta < T(0)) ta = T(0);
+            if (tb > T(1)) tb = T(1);
+            lb = ta;
+            ub = tb;
+        }
+    }
+}
+
 // Squared length of a segment as the decision kernel forms it
 // (.cu:1001-1005 for P, .cu:1138-1141 for Q).
 template <typename T>
@@ -183,7 +218,9 @@ T vefd_impl(const T *P, int64_t m, const T *Q, int64_t n, int D, T init) {
 // cells: rightLB to the next row (as leftLB), topLB along the row (as botLB)
 // (.cu:1231-1249).  Row 0 starts with botLB = 0, others with -1 (.cu:1026-1027);
 // the left boundary is -1 (.cu:656).  Returns rightUB of cell (m-2, n-2).
-template <typename T>
+// TWO: free intervals of the two-phase path (free_iv_2ph); the reachability
+// rules are the same in both paths (.cu:932-947 and .cu:1231-1244).
+template <typename T, bool TWO = false>
 T decide_rows(const T *P, int64_t m, const T *Q, int64_t n, int D, T r,
               int64_t r0, int64_t r1, std::vector<T> &colv,
               const std::vector<T> &A4b, const std::vector<T> &inv2b) {
@@ -200,8 +237,13 @@ T decide_rows(const T *P, int64_t m, const T *Q, int64_t n, int D, T r,
             T leftLB = (i == 0) ? T(-1) : colv[(size_t)j];
             const T *qa = Q + j * D, *qb = Q + (j + 1) * D;
             T rLB, rUB, tLB, tUB;
-            free_iv(pb, qa, qb, D, r2, A4b[(size_t)j], inv2b[(size_t)j], rLB, rUB);
-            free_iv(qb, pa, pb, D, r2, A4a, inv2a, tLB, tUB);
+            if (TWO) {
+                free_iv_2ph(pb, qa, qb, D, r2, A4b[(size_t)j], inv2b[(size_t)j], rLB, rUB);
+                free_iv_2ph(qb, pa, pb, D, r2, A4a, inv2a, tLB, tUB);
+            } else {
+                free_iv(pb, qa, qb, D, r2, A4b[(size_t)j], inv2b[(size_t)j], rLB, rUB);
+                free_iv(qb, pa, pb, D, r2, A4a, inv2a, tLB, tUB);
+            }
             if (botLB < T(0)) {
                 bool inv = leftLB > rUB || leftLB < T(0);
                 T t2 = std::fmax(rLB, leftLB);
@@ -235,12 +277,12 @@ void q_seg_terms(const T *Q, int64_t n, int D, std::vector<T> &A4b, std::vector<
     }
 }
 
-template <typename T>
+template <typename T, bool TWO = false>
 T decide_impl(const T *P, int64_t m, const T *Q, int64_t n, int D, T r) {
     std::vector<T> A4b, inv2b;
     q_seg_terms(Q, n, D, A4b, inv2b);
     std::vector<T> colv((size_t)(n - 1), T(-1));
-    return decide_rows(P, m, Q, n, D, r, 0, m - 1, colv, A4b, inv2b);
+    return decide_rows<T, TWO>(P, m, Q, n, D, r, 0, m - 1, colv, A4b, inv2b);
 }
 
 inline bool continue_bisect(double lb, double ub, int stop_mode, double eps) {
@@ -315,6 +357,26 @@ void or_free_interval(const double *P, const double *Q, int dims, int64_t i, int
     out[1] = ub;
 }
 
+void or_free_interval_2ph(const double *P, const double *Q, int dims, int64_t i, int64_t j,
+                          double r, int which, double *out) {
+    double lb, ub;
+    if (which == 0) {
+        const double *qa = Q + j * dims, *qb = Q + (j + 1) * dims;
+        double Ab = seg_len2(qa, qb, dims);
+        free_iv_2ph(P + (i + 1) * dims, qa, qb, dims, r * r, 4.0 * Ab, 0.5 / Ab, lb, ub);
+    } else {
+        const double *pa = P + i * dims, *pb = P + (i + 1) * dims;
+        double Aa = seg_len2(pa, pb, dims);
+        free_iv_2ph(Q + (j + 1) * dims, pa, pb, dims, r * r, 4.0 * Aa, 0.5 / Aa, lb, ub);
+    }
+    out[0] = lb;
+    out[1] = ub;
+}
+
+double or_decide_ub_2ph(const double *P, int64_t m, const double *Q, int64_t n, int dims, double r) {
+    return decide_impl<double, true>(P, m, Q, n, dims, r);
+}
+
 void or_ve_weights(const double *P, const double *Q, int dims, int64_t i, int64_t j,
                    double *out) {
     const double *pa = P + i * dims, *pb = P + (i + 1) * dims;
@@ -326,6 +388,13 @@ void or_ve_weights(const double *P, const double *Q, int dims, int64_t i, int64_
 int64_t or_distance(const double *P, int64_t m, const double *Q, int64_t n, int dims,
                     int stop_mode, double eps, int vefd_gate, double *out,
                     double *splits, int32_t *decisions, int64_t max_trace) {
+    return or_distance_form(P, m, Q, n, dims, stop_mode, eps, vefd_gate, 0, out, splits, decisions,
+                            max_trace);
+}
+
+int64_t or_distance_form(const double *P, int64_t m, const double *Q, int64_t n, int dims,
+                         int stop_mode, double eps, int vefd_gate, int form, double *out,
+                         double *splits, int32_t *decisions, int64_t max_trace) {
     double lb0 = or_endpoint_lb(P, m, Q, n, dims);
     double ub = or_dfd(P, m, Q, n, dims);
     double lb = lb0;
@@ -348,7 +417,9 @@ int64_t or_distance(const double *P, int64_t m, const double *Q, int64_t n, int 
     while (continue_bisect(lb, ub, stop_mode, eps)) {
         double split = (ub + lb) / 2.0;
         std::fill(colv.begin(), colv.end(), -1.0);
-        double rub = decide_rows<double>(P, m, Q, n, dims, split, 0, m - 1, colv, A4b, inv2b);
+        double rub = form == 1
+                         ? decide_rows<double, true>(P, m, Q, n, dims, split, 0, m - 1, colv, A4b, inv2b)
+                         : decide_rows<double>(P, m, Q, n, dims, split, 0, m - 1, colv, A4b, inv2b);
         int dec = rub == 1.0;
         if (splits && iter < max_trace) splits[iter] = split;
         if (decisions && iter < max_trace) decisions[iter] = dec;

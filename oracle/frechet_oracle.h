@@ -51,6 +51,15 @@ double or_decide_ub_f32(const double *P, int64_t m, const double *Q, int64_t n, 
 void or_free_interval(const double *P, const double *Q, int dims, int64_t i, int64_t j,
                       double r, int which, double *out);
 
+/* The two-phase decision path, interval_calculation<D> + reachability
+ * (.cu:674-958), which the reference runs when its four interval arrays fit
+ * in device memory (.cu:87-96, 1450-1558): the same reachability rules on
+ * intervals whose C term is (sum mi^2) - r2 and whose clamps keep a NaN end
+ * (zero-length segments).  Terminal rightUB, and the interval of one edge. */
+double or_decide_ub_2ph(const double *P, int64_t m, const double *Q, int64_t n, int dims, double r);
+void or_free_interval_2ph(const double *P, const double *Q, int dims, int64_t i, int64_t j,
+                          double r, int which, double *out);
+
 /* Vertex-edge weights of cell (i,j) (.cu:374-416, README.md:79-83):
  * out[0] = w_top = dist^2(q_{j+1}, p_i p_{i+1}), out[1] = w_right = dist^2(p_{i+1}, q_j q_{j+1}). */
 void or_ve_weights(const double *P, const double *Q, int dims, int64_t i, int64_t j,
@@ -67,6 +76,10 @@ void or_ve_weights(const double *P, const double *Q, int dims, int64_t i, int64_
 int64_t or_distance(const double *P, int64_t m, const double *Q, int64_t n, int dims,
                     int stop_mode, double eps, int vefd_gate, double *out,
                     double *splits, int32_t *decisions, int64_t max_trace);
+/* or_distance with the decision form: 0 = reachable<D> (fused), 1 = two-phase. */
+int64_t or_distance_form(const double *P, int64_t m, const double *Q, int64_t n, int dims,
+                         int stop_mode, double eps, int vefd_gate, int form, double *out,
+                         double *splits, int32_t *decisions, int64_t max_trace);
 
 /* Rows [r0, r1) of one sweep, for bounded CPU timing samples.  kind 0 = VE-FD,
  * 1 = decision at r, 2 = DFD.  Returns a checksum of the last row's outputs. */

@@ -35,7 +35,7 @@ def lib():
             getattr(L, name).restype = ctypes.c_double
         L.or_endpoint_lb.argtypes = [_dp, _i64, _dp, _i64, ctypes.c_int]
         L.or_endpoint_lb.restype = ctypes.c_double
-        for name in ("or_vefd", "or_vefd_f32", "or_decide_ub", "or_decide_ub_f32"):
+        for name in ("or_vefd", "or_vefd_f32", "or_decide_ub", "or_decide_ub_f32", "or_decide_ub_2ph"):
             getattr(L, name).argtypes = [_dp, _i64, _dp, _i64, ctypes.c_int, ctypes.c_double]
             getattr(L, name).restype = ctypes.c_double
         L.or_decide.argtypes = [_dp, _i64, _dp, _i64, ctypes.c_int, ctypes.c_double]
@@ -43,12 +43,18 @@ def lib():
         L.or_free_interval.argtypes = [_dp, _dp, ctypes.c_int, _i64, _i64, ctypes.c_double,
                                        ctypes.c_int, _dp]
         L.or_free_interval.restype = None
+        L.or_free_interval_2ph.argtypes = L.or_free_interval.argtypes
+        L.or_free_interval_2ph.restype = None
         L.or_ve_weights.argtypes = [_dp, _dp, ctypes.c_int, _i64, _i64, _dp]
         L.or_ve_weights.restype = None
         L.or_distance.argtypes = [_dp, _i64, _dp, _i64, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_double, ctypes.c_int, _dp, _dp,
                                   ctypes.POINTER(ctypes.c_int32), _i64]
         L.or_distance.restype = _i64
+        L.or_distance_form.argtypes = [_dp, _i64, _dp, _i64, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                       ctypes.POINTER(ctypes.c_int32), _i64]
+        L.or_distance_form.restype = _i64
         L.or_sweep_rows.argtypes = [_dp, _i64, _dp, _i64, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_double, _i64, _i64]
         L.or_sweep_rows.restype = ctypes.c_double
@@ -101,20 +107,28 @@ def vefd(P, Q, init_lb: float | None = None, fp32: bool = False) -> float:
     return f(_p(P), len(P), _p(Q), len(Q), P.shape[1], init_lb)
 
 
-def decide_ub(P, Q, r: float, fp32: bool = False) -> float:
+def decide_ub(P, Q, r: float, fp32: bool = False, form: int = 0) -> float:
+    """Terminal rightUB of the decision at r.  form 0: reachable<D>'s
+    intervals; form 1: the two-phase path's (interval_calculation<D>)."""
     P, Q = _c(P), _c(Q)
-    f = lib().or_decide_ub_f32 if fp32 else lib().or_decide_ub
+    if form == 1:
+        if fp32:
+            raise ValueError("the two-phase form is FP64")
+        f = lib().or_decide_ub_2ph
+    else:
+        f = lib().or_decide_ub_f32 if fp32 else lib().or_decide_ub
     return f(_p(P), len(P), _p(Q), len(Q), P.shape[1], r)
 
 
-def decide(P, Q, r: float, fp32: bool = False) -> bool:
-    return decide_ub(P, Q, r, fp32) == 1.0
+def decide(P, Q, r: float, fp32: bool = False, form: int = 0) -> bool:
+    return decide_ub(P, Q, r, fp32, form) == 1.0
 
 
-def free_interval(P, Q, i: int, j: int, r: float, which: int):
+def free_interval(P, Q, i: int, j: int, r: float, which: int, form: int = 0):
     P, Q = _c(P), _c(Q)
     out = np.zeros(2)
-    lib().or_free_interval(_p(P), _p(Q), P.shape[1], i, j, r, which, _p(out))
+    f = lib().or_free_interval_2ph if form == 1 else lib().or_free_interval
+    f(_p(P), _p(Q), P.shape[1], i, j, r, which, _p(out))
     return float(out[0]), float(out[1])
 
 
@@ -126,15 +140,16 @@ def ve_weights(P, Q, i: int, j: int):
 
 
 def distance(P, Q, stop_mode: int = 1, eps: float = 1e-9, vefd_gate: bool = True,
-             max_trace: int = 256):
-    """Returns dict(distance, endpoint_lb, dfd, vefd, lb, ub, iterations, splits, decisions)."""
+             max_trace: int = 256, form: int = 0):
+    """Returns dict(distance, endpoint_lb, dfd, vefd, lb, ub, iterations, splits, decisions).
+    form selects the decisions' interval form (0 fused, 1 two-phase)."""
     P, Q = _c(P), _c(Q)
     out = np.zeros(6)
     splits = np.zeros(max_trace)
     decs = np.zeros(max_trace, dtype=np.int32)
-    it = lib().or_distance(_p(P), len(P), _p(Q), len(Q), P.shape[1], stop_mode, eps,
-                           1 if vefd_gate else 0, _p(out), _p(splits),
-                           decs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), max_trace)
+    it = lib().or_distance_form(_p(P), len(P), _p(Q), len(Q), P.shape[1], stop_mode, eps,
+                                1 if vefd_gate else 0, form, _p(out), _p(splits),
+                                decs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), max_trace)
     k = min(it, max_trace)
     return dict(distance=out[0], endpoint_lb=out[1], dfd=out[2], vefd=out[3], lb=out[4],
                 ub=out[5], iterations=int(it), splits=splits[:k].copy(),

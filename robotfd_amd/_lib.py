@@ -20,6 +20,9 @@ STATUS_NAMES = {0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "no G
 
 STOP_ABS, STOP_REL = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
+# rfd_decide_form
+DECIDE_FUSED, DECIDE_TWO_PHASE, DECIDE_REFERENCE = 0, 1, 2
+DECIDE_FORMS = {"fused": DECIDE_FUSED, "two-phase": DECIDE_TWO_PHASE, "reference": DECIDE_REFERENCE}
 
 K_DFD, K_VEFD64, K_DEC64, K_VEFD32, K_DEC32, K_BATCH, K_MLP, K_OTHER, K_BOUNDS64 = range(9)
 KERNEL_NAMES = ["dfd", "vefd_f64", "decide_f64", "vefd_f32", "decide_f32", "batch", "mlp", "other",
@@ -32,7 +35,8 @@ class Opts(ctypes.Structure):
                 ("eps", ctypes.c_double), ("vefd_gate", ctypes.c_int32),
                 ("verbose", ctypes.c_int32), ("mixed_switch", ctypes.c_double),
                 ("rows_per_lane", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-                ("prune", ctypes.c_int32), ("spec_depth", ctypes.c_int32)]
+                ("prune", ctypes.c_int32), ("spec_depth", ctypes.c_int32),
+                ("decide_form", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):
@@ -77,6 +81,7 @@ SIGNATURES = {
     "rfd_session_destroy": (ctypes.c_int, [_vp]),
     "rfd_session_bounds": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, ctypes.c_int]),
     "rfd_session_decide": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _ip, _dp]),
+    "rfd_session_set_decide_form": (ctypes.c_int, [_vp, ctypes.c_int32, _ip]),
     "rfd_session_distance": (ctypes.c_int, [_vp, ctypes.POINTER(Opts), ctypes.POINTER(Result)]),
     "rfd_distance": (ctypes.c_int, [_vp, _dp, _i64, _dp, _i64, ctypes.c_int,
                                     ctypes.POINTER(Opts), ctypes.POINTER(Result)]),

@@ -55,9 +55,10 @@ class Batch:
 
     def run(self, *, stop: str = "rel", eps: float = 1e-9, vefd_gate: bool = True,
             rows_per_lane: int = 0, use_graph: bool = True, prune: int = -1,
-            spec_depth: int = 0) -> dict:
-        """Returns arrays: distance, dfd, vefd, lb, ub, iterations, latency_ms, ..."""
-        o = _opts(stop=stop, eps=eps, vefd_gate=vefd_gate)
+            spec_depth: int = 0, decide_form: str = "fused") -> dict:
+        """Returns arrays: distance, dfd, vefd, lb, ub, iterations, latency_ms, ...
+        decide_form: "fused", "two-phase" or "reference" (per pair)."""
+        o = _opts(stop=stop, eps=eps, vefd_gate=vefd_gate, decide_form=decide_form)
         o.rows_per_lane = rows_per_lane
         o.use_graph = 1 if use_graph else 0
         o.prune = prune

@@ -204,7 +204,11 @@ __global__ __launch_bounds__(64) void batch_sweep(BatchArgs a) {
         job.nbands = (int)((rows + 64LL * R - 1) / (64LL * R));
         job.param = (OP == OP_DEC) ? st.radius[node] : (OP == OP_VEFD ? p.endpoint_lb : 0.0);
         job.thr = (OP == OP_VEFD) ? st.thr : INFINITY;
-        run_band<OP, 3, double, SYNC_SCOPED, R, PRUNE>(job, band, qring, sring, evals, nullptr);
+        // A decision item runs the interval form of its pair (wave-uniform).
+        if (OP == OP_DEC && p.dec_form == DEC_TWO_PHASE)
+            run_band<OP, 3, double, SYNC_SCOPED, R, true>(job, band, qring, sring, evals, nullptr);
+        else
+            run_band<OP, 3, double, SYNC_SCOPED, R, PRUNE>(job, band, qring, sring, evals, nullptr);
     }
     if (a.evals) atomicAdd(a.evals, evals);
 }

@@ -39,7 +39,7 @@ struct BatchPair {
     long long bandstride;    // progress flags per tree node
     double endpoint_lb;      // frechet.cpp:30, computed on the host
     int big;                 // m*n > 10000 (.cu:1343, in 64 bits)
-    int pad;
+    int dec_form;            // decision interval form of this pair: DEC_FUSED or DEC_TWO_PHASE
 };
 
 struct BatchState {

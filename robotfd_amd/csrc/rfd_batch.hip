@@ -312,7 +312,7 @@ rfd_status rfd_batch_create(rfd_ctx *ctx, int64_t npairs, const double *const *P
         bofs += (long long)rfd::kSpec * p.bandstride;
         p.endpoint_lb = rfd_host::endpoint_bound(P[k], m[k], Q[k], n[k], dims[k]);
         p.big = (m[k] * n[k] > 10000) ? 1 : 0;
-        p.pad = 0;
+        p.dec_form = rfd::DEC_FUSED;
         cells_total += (double)m[k] * (double)n[k];
     }
     b->total_items = bofs;
@@ -403,6 +403,29 @@ rfd_status rfd_batch_tune(rfd_batch *b, const rfd_opts *opts, int32_t *rows_per_
     return RFD_OK;
 }
 
+// Each pair's decision form from opts->decide_form (RFD_DECIDE_REFERENCE:
+// the reference's free-memory rule for that pair's size); the pair records
+// are uploaded again when a form changes (the captured graph reads them from
+// device memory).
+static rfd_status apply_forms(rfd_batch *b, const rfd_opts &o) {
+    rfd_ctx *ctx = b->ctx;
+    if (o.decide_form < RFD_DECIDE_FUSED || o.decide_form > RFD_DECIDE_REFERENCE) return RFD_ERR_ARG;
+    bool changed = false;
+    for (auto &p : b->hpairs) {
+        int f = rfd::DEC_FUSED;
+        if (o.decide_form == RFD_DECIDE_TWO_PHASE) f = rfd::DEC_TWO_PHASE;
+        else if (o.decide_form == RFD_DECIDE_REFERENCE) f = rfd_host::reference_form(ctx, p.m, p.n);
+        if (p.dec_form != f) {
+            p.dec_form = f;
+            changed = true;
+        }
+    }
+    if (changed)
+        RFD_HIP(ctx, hipMemcpy(b->pairs, b->hpairs.data(), sizeof(rfd::BatchPair) * b->npairs,
+                               hipMemcpyHostToDevice));
+    return RFD_OK;
+}
+
 static rfd_status batch_run_impl(rfd_batch *b, const rfd_opts *opts, rfd_result *results,
                                  double *latency_ms, rfd_mlp *net, double tau, double delta,
                                  int32_t *certified) {
@@ -411,6 +434,7 @@ static rfd_status batch_run_impl(rfd_batch *b, const rfd_opts *opts, rfd_result 
     if (opts) o = *opts; else rfd_default_opts(&o);
     rfd_ctx *ctx = b->ctx;
     RFD_HIP(ctx, hipSetDevice(ctx->device));
+    if (rfd_status fs = apply_forms(b, o); fs != RFD_OK) return fs;
     int rpl = o.rows_per_lane;
     if (rpl < 1 || rpl > 4) {
         if (!b->tuned_rpl) {

@@ -38,6 +38,7 @@ struct rfd_session {
     unsigned long long *evals = nullptr; // cells evaluated by the last sweep
     double endpoint_lb = 0.0;
     double dfd_sq = -1.0;                // DFD^2 of this pair once known (prunes VE-FD)
+    int dec_form = 0;                    // rfd::DEC_FUSED or rfd::DEC_TWO_PHASE
 };
 
 namespace rfd {
@@ -84,11 +85,12 @@ void launch_sweep_rp(int dims, int grid, hipStream_t st, const rfd::SweepArgs &a
         hipLaunchKernelGGL((rfd::band_sweep<OP, 3, T, SYNC, R, PRUNE>), dim3(grid), dim3(64), 0, st, a);
 }
 
-// VE-FD with a finite threshold runs the pruning instantiation; every other
-// sweep runs without the per-cell test.
+// VE-FD with a finite threshold runs the pruning instantiation, and the
+// decision in the two-phase form its variant; every other sweep runs the plain
+// instantiation.
 template <int OP, typename T, int SYNC, int R>
 void launch_sweep_r(int dims, int grid, hipStream_t st, const rfd::SweepArgs &a) {
-    if (OP == rfd::OP_VEFD && std::isfinite(a.thr))
+    if ((OP == rfd::OP_VEFD && std::isfinite(a.thr)) || (OP == rfd::OP_DEC && a.dec_form == rfd::DEC_TWO_PHASE))
         launch_sweep_rp<OP, T, SYNC, R, true>(dims, grid, st, a);
     else
         launch_sweep_rp<OP, T, SYNC, R, false>(dims, grid, st, a);
@@ -168,6 +170,7 @@ rfd_status run_sweep(rfd_session *s, const T *P, const T *Q, const T *qseg, doub
     a.param = param;
     a.thr = thr;
     a.evals = s->evals;
+    a.dec_form = (OP == rfd::OP_DEC) ? s->dec_form : rfd::DEC_FUSED;
     const double sentinel = -3.0;  // .cu:193 (result = -3 until the terminal cell writes it)
     unsigned long long *tlog = nullptr;
     if (ctx->tlog_path) {
@@ -376,6 +379,32 @@ bool keep_bisecting(double lb, double ub, const rfd_opts &o, double eps) {
 }
 
 }  // namespace
+
+namespace rfd_host {
+
+// The reference's choice between its decision paths (.cu:87-96): two-phase
+// when its four interval arrays of ((m-1)(n+m-4) + m) doubles, plus
+// n_block * n_thread_per_block bytes, fit in free device memory; it reads
+// n_block = multiProcessorCount and n_thread_per_block = maxThreadsPerBlock
+// (.cu:34-36).
+int reference_form(rfd_ctx *ctx, int64_t m, int64_t n) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return rfd::DEC_FUSED;
+    hipDeviceProp_t prop;
+    int tpb = 1024;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) tpb = prop.maxThreadsPerBlock;
+    const long double need = (long double)ctx->cu_count * tpb +
+                             ((long double)(m - 1) * (long double)(n - 2 + m - 2) + (long double)m) * 8.0L * 4.0L;
+    return (long double)free_b < need ? rfd::DEC_FUSED : rfd::DEC_TWO_PHASE;
+}
+
+int resolve_form(rfd_session *s, int form) {
+    if (form == RFD_DECIDE_TWO_PHASE) return rfd::DEC_TWO_PHASE;
+    if (form == RFD_DECIDE_REFERENCE) return reference_form(s->ctx, s->m, s->n);
+    return rfd::DEC_FUSED;
+}
+
+}  // namespace rfd_host
 
 extern "C" {
 
@@ -612,7 +641,7 @@ rfd_status rfd_session_decide(rfd_session *s, double radius, int fp32, int *reac
         if ((st = ensure_f32(s)) != RFD_OK) return st;
         if ((st = ensure_qseg<float>(s, &s->qseg32, s->fQ)) != RFD_OK) return st;
         st = run_sweep<rfd::OP_DEC, float>(s, s->fP, s->fQ, s->qseg32, radius, RFD_K_DEC32, &ub);
-    } else if (ctx->fused && ctx->dec_cfg != 0) {
+    } else if (ctx->fused && ctx->dec_cfg != 0 && s->dec_form == rfd::DEC_FUSED) {
         // Opt-in (RFD_DEC): at 200k x 200k near the critical radius it takes
         // 410 ms against 217 ms for band_sweep, which skips the free
         // intervals of unreachable cells (about 80% of them there).
@@ -627,7 +656,16 @@ rfd_status rfd_session_decide(rfd_session *s, double radius, int fp32, int *reac
     return RFD_OK;
 }
 
-rfd_status rfd_session_distance(rfd_session *s, const rfd_opts *opts, rfd_result *res) {
+
+rfd_status rfd_session_set_decide_form(rfd_session *s, int32_t form, int32_t *effective) {
+    if (!s || form < RFD_DECIDE_FUSED || form > RFD_DECIDE_REFERENCE) return RFD_ERR_ARG;
+    RFD_HIP(s->ctx, hipSetDevice(s->ctx->device));
+    s->dec_form = rfd_host::resolve_form(s, form);
+    if (effective) *effective = s->dec_form;
+    return RFD_OK;
+}
+
+static rfd_status session_distance_impl(rfd_session *s, const rfd_opts *opts, rfd_result *res) {
     if (!s || !res) return RFD_ERR_ARG;
     rfd_opts o;
     if (opts) o = *opts; else rfd_default_opts(&o);
@@ -747,6 +785,21 @@ rfd_status rfd_session_distance(rfd_session *s, const rfd_opts *opts, rfd_result
         std::printf("Total Time: %f ms\n", (double)ms);
     }
     return RFD_OK;
+}
+
+rfd_status rfd_session_distance(rfd_session *s, const rfd_opts *opts, rfd_result *res) {
+    if (!s || !res) return RFD_ERR_ARG;
+    rfd_opts o;
+    if (opts) o = *opts; else rfd_default_opts(&o);
+    if (o.decide_form < RFD_DECIDE_FUSED || o.decide_form > RFD_DECIDE_REFERENCE) return RFD_ERR_ARG;
+    RFD_HIP(s->ctx, hipSetDevice(s->ctx->device));
+    // The call's decisions use opts->decide_form; the session's own form
+    // (rfd_session_set_decide_form) is restored afterwards.
+    const int saved = s->dec_form;
+    s->dec_form = rfd_host::resolve_form(s, o.decide_form);
+    const rfd_status st = session_distance_impl(s, &o, res);
+    s->dec_form = saved;
+    return st;
 }
 
 rfd_status rfd_distance(rfd_ctx *ctx, const double *P, int64_t m, const double *Q, int64_t n,

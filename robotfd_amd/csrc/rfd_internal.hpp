@@ -59,6 +59,10 @@ hipError_t rfd_launch_bounds(rfd_ctx *ctx, int dims, int cfg, int64_t nb, const 
 bool rfd_valid_batch_bounds_cfg(int cfg);
 void rfd_launch_batch_bounds(rfd_ctx *ctx, int cfg, hipStream_t st, const rfd::BatchArgs &a);
 
+// The reference's choice of decision path for an m x n pair from free device
+// memory (rfd_capi.hip): rfd::DEC_TWO_PHASE or rfd::DEC_FUSED.
+int reference_form(rfd_ctx *ctx, int64_t m, int64_t n);
+
 }  // namespace rfd_host
 
 #define RFD_HIP(ctx, call)                                                          \

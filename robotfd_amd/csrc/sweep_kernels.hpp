@@ -156,6 +156,41 @@ __device__ inline void free_iv(const T *c, const T *s0, const T *s1, T r2, T A4,
     }
 }
 
+// The same interval as the reference's two-phase decision path forms it
+// (interval_calculation<D>, .cu:722-782): the C term is (sum of mi^2) - r2,
+// the sum started at 0 (.cu:733-742), and the clamps are comparisons
+// (.cu:751-752), so a NaN end (0 * inf at a zero-length segment) stays NaN.
+template <typename T, int D>
+__device__ inline void free_iv_2ph(const T *c, const T *s0, const T *s1, T r2, T A4, T inv2A,
+                                   T &lb, T &ub) {
+    using N = Num<T>;
+    T B = T(0), mm = T(0);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        T di = s1[k] - s0[k];
+        T mi = s0[k] - c[k];
+        B = N::fma(mi, di, B);
+        mm = N::fma(mi, mi, mm);
+    }
+    B = B * T(2);
+    const T C = mm - r2;
+    T disc = N::fma(B, B, -(A4 * C));
+    lb = T(-1);
+    ub = T(-2);
+    if (disc >= T(0)) {
+        T s = N::sqrt(disc);
+        T ta = (-B - s) * inv2A;
+        T tb = (-B + s) * inv2A;
+        if (!(tb < T(0) || ta > T(1))) {
+            lb = (ta < T(0)) ? T(0) : ta;
+            ub = (tb > T(1)) ? T(1) : tb;
+        }
+    }
+}
+
+// Decision interval forms (SweepArgs::dec_form).
+enum DecForm { DEC_FUSED = 0, DEC_TWO_PHASE = 1 };
+
 struct SweepArgs {
     const void *P;       // AoS points of the row curve (T)
     const void *Q;       // AoS points of the column curve (T)
@@ -170,6 +205,7 @@ struct SweepArgs {
     double thr;          // VE-FD: prune threshold on incoming values (+inf: none)
     unsigned long long *evals;  // cells whose weights / free intervals were evaluated
     unsigned long long *tlog;   // optional, 4 per band: wave start, first input ready, end, hw id
+    int dec_form;        // decision: DEC_FUSED or DEC_TWO_PHASE (selects the kernel variant)
 };
 
 // "Dead" values: those that can no longer change the sweep's result.
@@ -250,7 +286,10 @@ template <int OP> struct Staged { static constexpr int value = (OP == OP_DEC) ? 
 // registers at step 64k and written to LDS at step 64(k+1), so global-memory
 // latency stays off the per-step path.  At step s lane l reads points s-l and
 // s-l+1: chunks k-1, k, k+1 are live, the fourth slot is free.
-template <int OP, int D, typename T, int SYNC, int R, bool PRUNE = false>
+// VAR selects the operation's variant: VE-FD skips cells whose incoming value
+// is at least thr (exact pruning); the decision uses the two-phase interval
+// form (free_iv_2ph).
+template <int OP, int D, typename T, int SYNC, int R, bool VAR = false>
 __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4][D][64],
                                 T (&sring)[4][Staged<OP>::value][64], unsigned long long &evals,
                                 unsigned long long *tlog) {
@@ -443,7 +482,7 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
                 } else if (OP == OP_VEFD) {
                     T in = N::min(left, bot[t]);
                     T Tv = in, Rv = in;
-                    if (!PRUNE || in < thr) {
+                    if (!VAR || in < thr) {
                         T wr = point_seg2<T, D>(p[t + 1], qa, qb);
                         T wt = point_seg2<T, D>(qb, p[t], p[t + 1]);
                         Tv = N::max(wt, in);
@@ -459,11 +498,19 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
                 } else {
                     const T leftLB = left, botLB = bot[t];
                     T rLB = T(-1), rUB = T(-2), tLB = T(-1);
-                    if (leftLB >= T(0) || botLB >= T(0)) {
+                    // A cell whose inputs are both unreachable yields (-1, -2, -1)
+                    // whatever its intervals; a NaN input (two-phase form) is not
+                    // unreachable, so the test is on < 0, not on >= 0.
+                    if (!(leftLB < T(0) && botLB < T(0))) {
                         ++evals;
                         T tUB;
-                        free_iv<T, D>(p[t + 1], qa, qb, r2, A4b, inv2b, rLB, rUB);
-                        free_iv<T, D>(qb, p[t], p[t + 1], r2, A4a[t], inv2a[t], tLB, tUB);
+                        if (VAR) {
+                            free_iv_2ph<T, D>(p[t + 1], qa, qb, r2, A4b, inv2b, rLB, rUB);
+                            free_iv_2ph<T, D>(qb, p[t], p[t + 1], r2, A4a[t], inv2a[t], tLB, tUB);
+                        } else {
+                            free_iv<T, D>(p[t + 1], qa, qb, r2, A4b, inv2b, rLB, rUB);
+                            free_iv<T, D>(qb, p[t], p[t + 1], r2, A4a[t], inv2a[t], tLB, tUB);
+                        }
                         if (botLB < T(0)) {
                             bool inv = leftLB > rUB || leftLB < T(0);
                             T t2 = N::max(rLB, leftLB);
@@ -507,7 +554,7 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
     }
 }
 
-template <int OP, int D, typename T, int SYNC, int R, bool PRUNE = false>
+template <int OP, int D, typename T, int SYNC, int R, bool VAR = false>
 __global__ __launch_bounds__(64) void band_sweep(SweepArgs a) {
     __shared__ T qring[4][D][64];
     __shared__ T sring[4][Staged<OP>::value][64];
@@ -520,7 +567,7 @@ __global__ __launch_bounds__(64) void band_sweep(SweepArgs a) {
         if (lane == 0) band = atomicAdd(a.queue, 1);
         band = __builtin_amdgcn_readfirstlane(band);
         if (band >= a.nbands) break;
-        run_band<OP, D, T, SYNC, R, PRUNE>(job, band, qring, sring, evals, a.tlog);
+        run_band<OP, D, T, SYNC, R, VAR>(job, band, qring, sring, evals, a.tlog);
     }
     if (a.evals) atomicAdd(a.evals, evals);
 }

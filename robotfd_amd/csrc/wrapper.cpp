@@ -44,7 +44,7 @@ namespace Frechet {
 namespace Continuous {
 
 double distance_cuda(const Curve &curve1, const Curve &curve2, const double eps,
-                     bool /*forceOversize*/) {
+                     bool forceOversize) {
     const double nan = std::numeric_limits<double>::quiet_NaN();
     if (curve1.empty() || curve2.empty()) {
         std::fprintf(stderr, "robotfd: empty curve\n");
@@ -64,6 +64,9 @@ double distance_cuda(const Curve &curve1, const Curve &curve2, const double eps,
     o.stop_mode = RFD_STOP_ABS;
     o.eps = eps;
     o.verbose = 1;
+    // The reference's decision path: reachable<D> when forceOversize is set
+    // or its interval arrays do not fit, else the two-phase path (.cu:87-96).
+    o.decide_form = forceOversize ? RFD_DECIDE_FUSED : RFD_DECIDE_REFERENCE;
     rfd_result r;
     rfd_status st = rfd_distance(ctx, P.data(), (int64_t)curve1.size(), Q.data(),
                                  (int64_t)curve2.size(), dims, &o, &r);

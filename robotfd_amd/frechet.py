@@ -134,9 +134,14 @@ class Context:
 
 
 def _opts(stop: str = "abs", eps: float = FLT_EPSILON, precision: str = "fp64",
-          vefd_gate: bool = True, verbose: bool = False, mixed_switch: float = 1e-4) -> _lib.Opts:
+          vefd_gate: bool = True, verbose: bool = False, mixed_switch: float = 1e-4,
+          decide_form: str = "fused") -> _lib.Opts:
+    """decide_form: "fused" (reachable<D>'s intervals), "two-phase"
+    (interval_calculation<D>'s) or "reference" (the reference's choice from
+    free device memory, per pair)."""
     o = _lib.Opts()
     lib().rfd_default_opts(ctypes.byref(o))
+    o.decide_form = _lib.DECIDE_FORMS[decide_form]
     o.stop_mode = {"abs": _lib.STOP_ABS, "rel": _lib.STOP_REL}[stop]
     o.precision = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}[precision]
     o.eps = float(eps)
@@ -155,8 +160,17 @@ class Intersection:
     (.cu:1398-1597); ``free_memory`` releases the device buffers (.cu:162-175).
     """
 
-    def __init__(self, curve1, curve2, forceOversize: bool = False, ctx: Context | None = None):
+    def __init__(self, curve1, curve2, forceOversize: bool = False, ctx: Context | None = None,
+                 decide_form: str = "fused"):
+        """decide_form fixes the session's decision path as the reference's
+        constructor does (.cu:87-96): "fused", "two-phase", or "reference"
+        (two-phase when the reference's interval arrays would fit in free
+        device memory and forceOversize is not set)."""
         self.ctx = ctx or Context.default()
+        self.decide_form = "fused" if (forceOversize and decide_form == "reference") else decide_form
+        if self.decide_form not in _lib.DECIDE_FORMS:
+            raise ValueError(f"decide_form must be one of {sorted(_lib.DECIDE_FORMS)}")
+        self.effective_form = None
         self.P = _lib.as_curve(curve1.array() if isinstance(curve1, Curve) else curve1)
         self.Q = _lib.as_curve(curve2.array() if isinstance(curve2, Curve) else curve2)
         if self.P.shape[1] != self.Q.shape[1]:
@@ -171,6 +185,17 @@ class Intersection:
                                        len(self.Q), self.P.shape[1], ctypes.byref(h)),
               self.ctx.handle)
         self.handle = h
+        self.set_decide_form(self.decide_form)
+
+    def set_decide_form(self, form: str) -> str:
+        """Decision path of intersection_interval_call_gpu / decide_ub; returns
+        the form in use ("fused" or "two-phase")."""
+        self._need()
+        eff = ctypes.c_int(0)
+        check(lib().rfd_session_set_decide_form(self.handle, _lib.DECIDE_FORMS[form], ctypes.byref(eff)),
+              self.ctx.handle)
+        self.effective_form = "two-phase" if eff.value == _lib.DECIDE_TWO_PHASE else "fused"
+        return self.effective_form
 
     def _need(self):
         if self.handle is None:
@@ -225,13 +250,15 @@ class Intersection:
 
 
 def distance(P, Q, *, stop: str = "rel", eps: float = 1e-9, precision: str = "fp64",
-             vefd_gate: bool = True, verbose: bool = False, ctx: Context | None = None) -> dict:
+             vefd_gate: bool = True, verbose: bool = False, ctx: Context | None = None,
+             decide_form: str = "fused") -> dict:
     """Full distance computation on two AoS arrays; returns the rfd_result fields."""
     ctx = ctx or Context.default()
     P = _lib.as_curve(P)
     Q = _lib.as_curve(Q)
     res = _lib.Result()
-    o = _opts(stop=stop, eps=eps, precision=precision, vefd_gate=vefd_gate, verbose=verbose)
+    o = _opts(stop=stop, eps=eps, precision=precision, vefd_gate=vefd_gate, verbose=verbose,
+              decide_form=decide_form)
     check(lib().rfd_distance(ctx.handle, dptr(P), len(P), dptr(Q), len(Q), P.shape[1],
                              ctypes.byref(o), ctypes.byref(res)), ctx.handle)
     return res.as_dict()
@@ -239,9 +266,11 @@ def distance(P, Q, *, stop: str = "rel", eps: float = 1e-9, precision: str = "fp
 
 def distance_cuda(curve1: Curve, curve2: Curve, eps: float, forceOversize: bool = False,
                   verbose: bool = True) -> float:
-    """frechet.cpp:18-75: bracket [VE-FD, DFD], bisection while ub - lb > eps."""
-    return distance(curve1.array(), curve2.array(), stop="abs", eps=eps,
-                    verbose=verbose)["distance"]
+    """frechet.cpp:18-75: bracket [VE-FD, DFD], bisection while ub - lb > eps,
+    with the reference's decision path (fused when forceOversize is set or its
+    interval arrays would not fit, else two-phase; .cu:87-96)."""
+    return distance(curve1.array(), curve2.array(), stop="abs", eps=eps, verbose=verbose,
+                    decide_form="fused" if forceOversize else "reference")["distance"]
 
 
 def compute_distance_parallel(curve1: Curve, curve2: Curve, forceOversize: bool = False) -> float:

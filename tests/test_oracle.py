@@ -343,3 +343,72 @@ def test_sweep_rows_mt_matches_single_thread(kind, threads):
         st = O.sweep_rows(P, Q, kind, r, r0, r1)
         mt = O.sweep_rows_mt(P, Q, kind, r, r0, r1, threads)
         assert mt == st, (r0, r1)
+
+
+# ---------------- two-phase decision form ----------------
+
+@pytest.mark.parametrize("seed", range(6))
+def test_two_phase_decision_away_from_critical_values(seed):
+    """interval_calculation + reachability (.cu:674-958) decides like the
+    fused reachable<D> and like textbook Alt-Godau at every radius that is not
+    within 1e-9 (relative) of a critical value, where rounding may differ."""
+    rng = np.random.default_rng(500 + seed)
+    checked = 0
+    for _ in range(25):
+        m, n = rng.integers(2, 8, 2)
+        d = 2 + seed % 2
+        P, Q = walk(rng, m, d), walk(rng, n, d)
+        crit = I.critical_values(P, Q)
+        # From the endpoint bound up: the sweep takes cell (0,0) as reachable
+        # (.cu:1026) and the bisection never asks below that bound.
+        for r in np.linspace(O.endpoint_lb(P, Q), O.dfd(P, Q) * 1.1, 13):
+            if np.min(np.abs(crit - r)) <= 1e-9 * r:
+                continue
+            two = O.decide(P, Q, r, form=1)
+            assert two == O.decide(P, Q, r)
+            assert two == I.decide_altgodau(P, Q, r)
+            checked += 1
+    assert checked > 150
+
+
+def test_two_phase_intervals_differ_from_fused_in_rounding_only():
+    """The C term is (sum mi^2) - r^2 (.cu:733-742) instead of a sum started at
+    -r^2 (.cu:1176-1183): some interval ends round differently, by a few ulps,
+    never by more."""
+    rng = np.random.default_rng(77)
+    differ = 0
+    for _ in range(60):
+        P, Q = walk(rng, 6, 3), walk(rng, 6, 3)
+        for i in range(5):
+            for j in range(5):
+                for which in (0, 1):
+                    r = rng.uniform(0.3, 3.0)
+                    a = O.free_interval(P, Q, i, j, r, which)
+                    b = O.free_interval(P, Q, i, j, r, which, form=1)
+                    if a != b:
+                        differ += 1
+                        assert (a[0] < 0) == (b[0] < 0)
+                        assert a == pytest.approx(b, rel=1e-12, abs=1e-12)
+    assert differ > 0
+
+
+def test_two_phase_zero_length_segment_keeps_nan():
+    """A zero-length segment makes 0.5/|ab|^2 infinite and t = 0 * inf NaN.
+    The two-phase clamps are comparisons (.cu:751-752, 778-779), so the
+    interval stays NaN; reachable<D>'s max / min give [0, 1]."""
+    P = np.array([[0.0, 0.0], [1.0, 0.0], [1.0, 0.0], [2.0, 0.0]])
+    Q = np.array([[0.0, 1.0], [2.0, 1.0]])
+    for r in (0.5, 1.5):
+        lb, ub = O.free_interval(P, Q, 1, 0, r, 1, form=1)
+        assert math.isnan(lb) and math.isnan(ub)
+        assert O.free_interval(P, Q, 1, 0, r, 1) == (0.0, 1.0)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_two_phase_distance_within_tolerance(seed):
+    rng = np.random.default_rng(900 + seed)
+    P, Q = walk(rng, 40, 2), walk(rng, 55, 2)
+    a = O.distance(P, Q, stop_mode=1, eps=1e-9)
+    b = O.distance(P, Q, stop_mode=1, eps=1e-9, form=1)
+    assert b["dfd"] == a["dfd"] and b["vefd"] == a["vefd"]
+    assert b["distance"] == pytest.approx(a["distance"], rel=2e-9)
