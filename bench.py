@@ -62,10 +62,10 @@ def sweep_mix(stats: dict, m: int, n: int) -> dict:
     return mix
 
 
-def measured_traffic(m: int, n: int, kernel: str):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE runs (scaled by the copy-kernel calibration), when
-    they were taken on this workload; None otherwise."""
+def _profiled(m: int, n: int, kernel: str):
+    """The dominant kernel's entry of the committed rocprofv3 counter runs
+    (profiles/hbm_traffic_band_sweep_200k.json), when they were taken on this
+    workload; None otherwise."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_band_sweep_200k.json")
     if (m, n) != (200_000, 200_000) or not os.path.exists(path):
         return None
@@ -75,8 +75,23 @@ def measured_traffic(m: int, n: int, kernel: str):
            "bounds_f64": KERNEL_SYMBOL["bounds_f64"].replace("rfd::", "")}.get(kernel)
     for name, v in data["kernels"].items():
         if tag and tag in name:
-            return v["traffic_bytes"]
+            return v
     return None
+
+
+def measured_traffic(m: int, n: int, kernel: str):
+    """HBM bytes per launch of the dominant kernel from the committed FETCH_SIZE
+    / WRITE_SIZE runs, scaled by the copy-kernel calibration; None if absent."""
+    v = _profiled(m, n, kernel)
+    return v["traffic_bytes"] if v else None
+
+
+def measured_valu_issue(m: int, n: int, kernel: str):
+    """Fraction of the busy CUs' VALU issue cycles the kernel used, from the
+    committed SQ_INSTS_VALU / GRBM_GUI_ACTIVE run (4 cycles per wave64 VALU
+    instruction); None if absent."""
+    v = _profiled(m, n, kernel)
+    return v.get("valu_issue_frac") if v else None
 
 
 def cpu_threads() -> int:
@@ -317,6 +332,9 @@ def main() -> int:
             "flops_per_launch": flops,
             "cells_per_launch": st["cells"] / st["launches"],
             "evaluated_cells_per_launch": st["evaluated"] / st["launches"]}
+    if roof["traffic"]:
+        roof["hbm_gbs"] = roof["traffic"] / (avg_ms * 1e-3) / 1e9  # profiled bytes over this run's launch time
+    roof["valu_issue_frac"] = measured_valu_issue(m, n, dom)
 
     out = {
         "metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
