@@ -25,7 +25,7 @@ struct rfd_ctx {
     int bounds_cfg = 0;           // fused sweep: 10 * rows per lane + producer waves, 0 = by size (RFD_BOUNDS)
     int wg_per_cu = 0;            // fused sweep: cap on workgroups per CU, 0 = none (RFD_WGCU)
     int dec_cfg = 0;              // fused decision sweep configuration, 0 = off (RFD_DEC opts in)
-    int batch_bounds = 137;       // batches: fused bound round configuration, 0 = separate rounds (RFD_BATCH_BOUNDS)
+    int batch_bounds = 1310;      // batches: fused bound round configuration, 0 = separate rounds (RFD_BATCH_BOUNDS)
     int lds_per_cu = 0;           // bytes of LDS per CU
     const char *tlog_path = nullptr;  // RFD_TLOG: append per-band timestamps of every sweep
     rfd_stats stats{};
