@@ -34,12 +34,12 @@
 // differences p_i - q_{j-1} from the step before.
 //
 // Hand-offs.  Inside the workgroup: producer w publishes the count of steps it
-// has written (LDS word, after s_waitcnt lgkmcnt(0)); the chain publishes the
-// count of steps it has consumed; a producer writes step s only once the chain
-// has consumed step s - KS.  Between bands: as band_sweep (scoped atomics,
-// in-order band queue), with two values per column (DFD, VE-FD), but carried
-// by a third role, the courier wave, through two LDS rings, so that the chain
-// never waits on global memory.
+// has written (an LDS word stored after its weights: a wave's LDS operations
+// complete in order); the chain publishes the count of steps it has finished;
+// a producer writes step s only once the chain has finished step s - KS.
+// Between bands: as band_sweep (scoped atomics, in-order band queue), with two
+// values per column (DFD, VE-FD), but carried by a third role, the courier
+// wave, through two LDS rings, so that the chain never waits on global memory.
 #pragma once
 
 #include <type_traits>
@@ -66,14 +66,37 @@ struct BoundsArgs {
 __device__ inline int lds_get(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 __device__ inline void lds_put(int *p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 
-// Ring depth in steps, 1536 R C bytes each (C columns per step): at most 43 KB
-// per workgroup for the configurations with few producers (three workgroups
-// per CU with the other rings), about 120 KB for those with five or more (one
-// workgroup per CU).
+// Columns of Q staged in LDS for the producers: 4C chunks of 64 columns,
+// component-major (component k of column j at k * qring_cols(C) + (j mod
+// qring_cols(C))).
+constexpr int qring_cols(int C) { return 256 * C; }
+
+// LDS plan of one workgroup.  The weight ring holds KS steps of 1536 R C bytes
+// (C columns per step), as many as fit (at most 16) beside the Q ring, the
+// boundary rings and the hand-off words: within 160 KB for the configurations
+// with five or more producers (one workgroup per CU), within a third of that
+// otherwise (three workgroups per CU).  The chain's step loop runs in trips
+// of KS steps, one per ring slot; the boundary rings hold KB positions, a
+// multiple of C KS (so that a trip's positions never wrap), at least 128
+// (64 with few producers, whose steps are short).
 template <int R, int NPW, int C = 1> struct BoundsRing {
-    static constexpr int budget = (NPW < 5) ? 44032 : 122880;
-    static constexpr int KS = (budget / (1536 * R * C) < 16) ? budget / (1536 * R * C) : 16;
-    static_assert(KS >= 2, "ring needs two slots");
+    static constexpr int kb(int ks) {
+        const int min_cols = (NPW < 5) ? 64 : 128;
+        return ((min_cols + C * ks - 1) / (C * ks)) * (C * ks);
+    }
+    static constexpr int bytes(int ks) {
+        return ks * 1536 * R * C + 3 * qring_cols(C) * 8 + 2 * 2 * kb(ks) * 8 + 2 * (64 + C * ks) * 8 + 64 * 4 +
+               16 * 4 + 64;
+    }
+    static constexpr int cap = (NPW < 5) ? 163840 / 3 : 163840;
+    static constexpr int pick() {
+        int k = 16;
+        while (k > 2 && bytes(k) > cap) --k;
+        return k;
+    }
+    static constexpr int KS = pick();
+    static constexpr int KB = kb(KS);
+    static_assert(bytes(KS) <= cap, "ring needs two slots");
 };
 
 __device__ inline double dpp_f64(double old, double src, int ctrl_shl) {
@@ -166,9 +189,10 @@ static __device__ inline double dmax(double a, double b) {
 
 // Row groups of the chain's software pipeline: NG groups of at most GMAX
 // rows, group g holding rows start(g) .. start(g + 1) - 1.  Three groups at
-// one column per step; groups of at most three rows at two.
-template <int R, int C> struct ChainPlan {
-    static constexpr int GT = (C == 1) ? (R + 2) / 3 : 3;
+// one column per step; groups of at most three rows at two, or where the
+// registers are tight (G = 3: four waves per SIMD, 128 VGPRs).
+template <int R, int C, int G = 0> struct ChainPlan {
+    static constexpr int GT = (G > 0) ? G : (C == 1) ? (R + 2) / 3 : 3;
     static constexpr int NG = (R + GT - 1) / GT;
     static constexpr int GMAX = (R + NG - 1) / NG;
     static constexpr int start(int g) { return (R * g) / NG; }
@@ -177,9 +201,9 @@ template <int R, int C> struct ChainPlan {
 
 // Weights of group g for the C columns of one step, issued as one batch of LDS
 // loads.  A slot holds [column][row][d, wr, wt][lane].
-template <int R, int C, int g>
-__device__ inline void chain_load(const double *w, double (&wv)[C][ChainPlan<R, C>::GMAX][3]) {
-    using PL = ChainPlan<R, C>;
+template <int R, int C, int G, int g>
+__device__ inline void chain_load(const double *w, double (&wv)[C][ChainPlan<R, C, G>::GMAX][3]) {
+    using PL = ChainPlan<R, C, G>;
 #pragma unroll
     for (int c = 0; c < C; ++c)
 #pragma unroll
@@ -191,10 +215,10 @@ __device__ inline void chain_load(const double *w, double (&wv)[C][ChainPlan<R, 
 // Orders a group's results before any later memory access: the next loads
 // into the group's registers must not be hoisted above the compute that
 // frees them.
-template <int R, int C, int g>
+template <int R, int C, int G, int g>
 __device__ inline void chain_done(const double (&dbot)[R], const double (&vbot)[R], const double (&dl)[C],
                                   const double (&vl)[C]) {
-    using PL = ChainPlan<R, C>;
+    using PL = ChainPlan<R, C, G>;
 #pragma unroll
     for (int c = 0; c < C; ++c) asm volatile("" ::"v"(dl[c]), "v"(vl[c]) : "memory");
 #pragma unroll
@@ -208,10 +232,10 @@ __device__ inline void chain_done(const double (&dbot)[R], const double (&vbot)[
 // own values from the previous column:
 //   DFD   (.cu:458-641)  top = max(d, min(dl, dbot)),  dl' = min(top, dbot)
 //   VE-FD (.cu:200-432)  in = min(vl, vbot),  T = max(wt, in),  vl' = R = max(wr, in)
-template <int R, int C, int g>
-__device__ inline void chain_rows(const double (&wv)[C][ChainPlan<R, C>::GMAX][3], double (&dbot)[R],
+template <int R, int C, int G, int g>
+__device__ inline void chain_rows(const double (&wv)[C][ChainPlan<R, C, G>::GMAX][3], double (&dbot)[R],
                                   double (&vbot)[R], double (&dl)[C], double (&vl)[C]) {
-    using PL = ChainPlan<R, C>;
+    using PL = ChainPlan<R, C, G>;
     // One wait for the whole group (its loads were issued a group earlier),
     // not one counter wait per load.
 #pragma unroll
@@ -238,10 +262,7 @@ __device__ inline void chain_rows(const double (&wv)[C][ChainPlan<R, C>::GMAX][3
         }
 }
 
-// Columns of the band-boundary rings in LDS (incoming and outgoing).
-constexpr int kRing = 128;
-
-// LDS words shared by the waves of one workgroup.
+// LDS words of the fused decision sweep's hand-offs (dec_kernels.hpp).
 struct BoundsSync {
     int *prod;       // [NPW] steps each producer has written
     int *cons;       // steps the chain has consumed
@@ -250,29 +271,74 @@ struct BoundsSync {
     int *out_done;   // outgoing columns the courier has published
 };
 
-// Calls f(integral_constant<int, g>) for g = G .. ChainPlan<R, C>::NG - 1.
-template <int R, int C, int G, typename F>
-__device__ inline void for_groups(F &&f) {
-    if constexpr (G < ChainPlan<R, C>::NG) {
-        f(std::integral_constant<int, G>{});
-        for_groups<R, C, G + 1>(f);
+// Calls f(integral_constant<int, i>) for i = I .. N - 1.
+template <int I, int N, typename F>
+__device__ inline void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
     }
 }
 
-template <int R, int KS, int C>
-__device__ inline void bounds_chain(const BoundsArgs &a, const int band, const double *wring, double *inring,
-                                    double *outring, const BoundsSync &sy, const int npw) {
-    using PL = ChainPlan<R, C>;
+// LDS of one workgroup of the fused sweep.  The boundary rings hold, per
+// position, the DFD and VE-FD values of one column: incoming column c at
+// position c mod KB, outgoing column c (lane 63's output at step c / C + 63)
+// at position (c + 63 C) mod KB, so that a step's positions follow its index
+// and a trip's never wrap.  Lanes 0-62 of the chain store their outputs to
+// slots of their own (odummy), so that the chain's store needs no lane mask.
+template <int R, int NPW, int C = 1> struct BoundsShared {
+    static constexpr int KS = BoundsRing<R, NPW, C>::KS;
+    static constexpr int KB = BoundsRing<R, NPW, C>::KB;
+    // The small arrays first, so that their addresses fit a DS instruction's
+    // 16-bit offset.
+    alignas(16) int sprod[16];  // steps each producer has written
+    int cons[64];               // steps the chain has finished (one copy per lane; read [0])
+    int in_avail;               // incoming columns in the ring
+    int out_done;               // outgoing columns the courier has published
+    int item;
+    alignas(16) double inring[2 * KB];
+    alignas(16) double outring[2 * KB];
+    alignas(16) double odummy[2 * (64 + C * KS)];
+    alignas(16) double qring[3 * qring_cols(C)];
+    alignas(16) double wring[KS * C * R * 3 * 64];
+};
+
+// Steps of band `band`.  Lane l evaluates columns C (s - l) .. C (s - l) + C - 1
+// at step s, so a band takes n / C + 63 steps.  The band holding row m - 1
+// (the last band, which has no successor) stops at the step where that row's
+// lane reaches column n - 1.
+template <int R, int C>
+__device__ inline int band_steps(const BoundsArgs &a, const int band) {
+    const int cols = (int)a.cols;
+    const long long toff = a.rows - 1 - (long long)band * 64 * R;
+    return (toff >= 0 && toff < 64LL * R) ? cols / C + (int)(toff / R) : cols / C + 63;
+}
+
+template <int R, int NPW, int C>
+__device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C> &sh) {
+    constexpr int GP = (NPW + 2 > 12) ? 3 : 0;  // four waves per SIMD: 128 VGPRs
+    using PL = ChainPlan<R, C, GP>;
     constexpr int NG = PL::NG;
+    constexpr int KS = BoundsShared<R, NPW, C>::KS;
+    constexpr int KB = BoundsShared<R, NPW, C>::KB;
+    constexpr int SLOT = C * R * 3 * 64;  // doubles per ring slot
+    constexpr int NONE = 0x3fffffff;      // no limit
     static_assert(NG >= 2, "the pipeline needs two row groups");
     const int lane = (int)(threadIdx.x & 63);
     const double big = (double)FLT_MAX;
     // Steps and columns in 32 bits (the launch requires n < 2^31 - 64 and, at
     // two columns per step, n even), so that the loop's bookkeeping stays on
-    // the scalar unit.  Lane l evaluates columns C(s - l) .. C(s - l) + C - 1
-    // at step s.
+    // the scalar unit.
     const int cols = (int)a.cols;
-    const int nsteps = cols / C + 63;
+    const int nsteps = band_steps<R, C>(a, band);
+    const bool has_succ = band + 1 < a.nbands;
+    // Terminal cell (m-1, n-1): when this band holds row m - 1, it is row tl
+    // of lane tlane, which reaches column n - 1 (the last column of a step) at
+    // the band's last step.
+    const long long toff = a.rows - 1 - (long long)band * 64 * R;
+    const bool has_term = toff >= 0 && toff < 64LL * R;
+    const int tlane = has_term ? (int)(toff / R) : -1;
+    const int tl = has_term ? (int)(toff - (long long)tlane * R) : 0;
     double dbot[R], vbot[R];
 #pragma unroll
     for (int t = 0; t < R; ++t) {
@@ -282,93 +348,108 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, const d
     double dout[C], vout[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) dout[c] = vout[c] = big;
-    const bool has_pred = band > 0;
-    const bool has_succ = band + 1 < a.nbands;
-    // Terminal cell (m-1, n-1): when this band holds row m - 1, it is row tl
-    // of lane tlane, which reaches column n - 1 (the last column of a step) at
-    // step n / C - 1 + tlane.
-    const long long toff = a.rows - 1 - (long long)band * 64 * R;
-    const bool has_term = toff >= 0 && toff < 64LL * R;
-    const int tlane = has_term ? (int)(toff / R) : -1;
-    const int tl = has_term ? (int)(toff - (long long)tlane * R) : 0;
-    const int s_term = has_term ? cols / C - 1 + tlane : -1;
-    int avail = 0, in_known = 0, done_known = 0;
-    int slot = 0;
     if (a.tlog && lane == 0) a.tlog[4 * band] = wall_clock64();
-    // The first band's row -1 (lane 0's input from below) holds DFD's 0 at
-    // column 0 and +big after it, VE-FD's +big: two entries of its own ring.
-    if (!has_pred && lane == 0) {
-        inring[0] = 0.0;
-        inring[1] = big;
-        inring[kRing] = big;
-        inring[kRing + 1] = big;
-    }
     // The chain is the band's serial path and every later band waits on it:
     // its SIMD issues for it first, producers fill the remaining cycles.
     __builtin_amdgcn_s_setprio(3);
+
+    // Hand-offs.  Step s reads its weights (loaded a step earlier) and loads
+    // step s + 1's: the producers must have written min(s + 2, nsteps) steps.
+    // It loads step s + 1's input from below: the courier must have filled
+    // min(C (s + 2), n) incoming columns.  It stores lane 63's columns
+    // C (s - 63) .. into the outgoing ring, over the columns KB earlier: the
+    // courier must have published C (s - 62) - KB.  Steps below `lim` meet all
+    // three by the counts last read, so a step checks one scalar; at `lim` the
+    // chain reads all the counts (waiting while one falls short) and moves it.
+    int avail = 0, in_known = 0, out_known = 0;
+    auto read_prod = [&]() __attribute__((always_inline)) {
+        asm volatile("" ::: "memory");  // read the words again on every poll
+        int mn = sh.sprod[0];
+#pragma unroll
+        for (int k = 1; k < NPW; ++k) mn = sh.sprod[k] < mn ? sh.sprod[k] : mn;
+        return __builtin_amdgcn_readfirstlane(mn);
+    };
+    auto wait_for = [&](const int s) __attribute__((always_inline)) {
+        const int np = (s + 2 < nsteps) ? s + 2 : nsteps;
+        const int ni = (C * (s + 2) < cols) ? C * (s + 2) : cols;
+        const int no = C * (s - 62) - KB;
+        for (;;) {
+            avail = read_prod();
+            in_known = __builtin_amdgcn_readfirstlane(lds_get(&sh.in_avail));
+            out_known = __builtin_amdgcn_readfirstlane(lds_get(&sh.out_done));
+            if (avail >= np && in_known >= ni && (!has_succ || out_known >= no)) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    };
+    auto limit = [&]() __attribute__((always_inline)) {
+        const int lp = (avail >= nsteps) ? NONE : avail - 1;
+        const int li = (in_known >= cols) ? NONE : in_known / C - 1;
+        const int lo = has_succ ? (out_known + KB) / C + 63 : NONE;
+        const int m = lp < li ? lp : li;
+        return __builtin_amdgcn_readfirstlane(m < lo ? m : lo);
+    };
 
     // Software pipeline over the row groups: a group's LDS loads are issued
     // while the group before it computes (group 0's, with the input from
     // below, while the last group of the step before computes), so no LDS
     // round trip sits between two steps' min/max chains, and two groups'
     // weights occupy registers at a time.  Loads are issued on every path
-    // (the last step reads a slot and ring entries nobody uses), so that the
-    // compiler's LDS counter waits stay exact.  Polls, which wait for all of
-    // this wave's LDS operations, come right after a group's compute.
+    // (the last step reads a slot and ring positions nobody uses), so that
+    // the compiler's LDS counter waits stay exact.
     double w[NG][C][PL::GMAX][3];
     double bD[C], bV[C];  // the next step's input from below (lane 0)
-    auto wait_pred = [&](const int st) __attribute__((always_inline)) {
-        // Row 64Rb - 1 at columns C st .. C st + C - 1.
-        const int need = (C * st + C < cols) ? C * st + C : cols;
-        if (has_pred && need > in_known) {
-            for (;;) {
-                in_known = __builtin_amdgcn_readfirstlane(lds_get(sy.in_avail));
-                if (need <= in_known) break;
-                __builtin_amdgcn_s_sleep(1);
-            }
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            if (a.tlog && lane == 0 && st == 0) a.tlog[4 * band + 1] = wall_clock64();
-        }
-    };
-    auto wait_prod = [&](const int st) __attribute__((always_inline)) {
-        if (st >= avail) {
-            for (;;) {
-                int mn = lds_get(&sy.prod[0]);
-                for (int k = 1; k < npw; ++k) {
-                    const int v = lds_get(&sy.prod[k]);
-                    mn = v < mn ? v : mn;
-                }
-                avail = __builtin_amdgcn_readfirstlane(mn);
-                if (st < avail) break;
-                __builtin_amdgcn_s_sleep(1);
-            }
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        }
-    };
-    auto load_in = [&](const int st) __attribute__((always_inline)) {
+    auto load_in = [&](const double *p) __attribute__((always_inline)) {
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            int col = C * st + c;
-            col = col < cols ? col : cols - 1;
-            const int r = has_pred ? (col & (kRing - 1)) : (col < 1 ? col : 1);
-            bD[c] = inring[r];
-            bV[c] = inring[kRing + r];
+            bD[c] = p[2 * c];
+            bV[c] = p[2 * c + 1];
         }
     };
-    wait_pred(0);
-    wait_prod(0);
-    load_in(0);
-    chain_load<R, C, 0>(wring + lane, w[0]);
+    wait_for(-1);
+    if (a.tlog && lane == 0 && band > 0) a.tlog[4 * band + 1] = wall_clock64();
+    int lim = limit();
+    // Ring slots through NBASE base registers, NB slots each, so that every
+    // offset fits a DS instruction's 16-bit field (the compiler folds no
+    // larger one).  The offsets are hidden from it, or it would fold them back
+    // into one base.
+    constexpr int SLOTB = SLOT * 8;
+    constexpr int NB = 65536 / SLOTB;
+    constexpr int NBASE = (KS + NB - 1) / NB;
+    const double *wb[NBASE];
+#pragma unroll
+    for (int i = 0; i < NBASE; ++i) {
+        int off = i * NB * SLOT;
+        asm volatile("" : "+v"(off));
+        wb[i] = sh.wring + lane + off;
+    }
+    // Slot k: a compile-time k (integral_constant) through its base register,
+    // a run-time one (the last steps) directly.
+    auto slot_ptr = [&](auto k) __attribute__((always_inline)) {
+        if constexpr (std::is_integral_v<decltype(k)>) {
+            return (const double *)(sh.wring + lane + k * SLOT);
+        } else {
+            constexpr int kk = decltype(k)::value;
+            return wb[kk / NB] + (kk % NB) * SLOT;
+        }
+    };
+    chain_load<R, C, GP, 0>(slot_ptr(std::integral_constant<int, 0>{}), w[0]);
+    load_in(sh.inring);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    // Steps finished, in a vector register: the store of the count then needs
+    // no copy from a scalar one.
+    int vdone = 0;
+    asm volatile("" : "+v"(vdone));
 
-    // Every lane runs every step: before its first column the producers'
-    // weights keep its rows at their initial values, and after its last
-    // column it computes values nobody reads (the terminal cell is written at
-    // its own step, and lane 63's hand-off columns never pass n - 1).
-    for (int s = 0; s < nsteps; ++s) {
-        const int nslot = (slot + 1 == KS) ? 0 : slot + 1;
-        const double *wcur = wring + slot * (C * R * 3 * 64) + lane;
-        const double *wnext = wring + nslot * (C * R * 3 * 64) + lane;
+    // Step s0 + k from ring slot k; the next step's weights come from slot
+    // kn and its input from below from in_next; `ow` is this step's output
+    // slot; lrel = lim - s0.  Every lane runs every step: before its first
+    // column the producers' weights keep its rows at their initial values,
+    // and after its last column it computes values nobody reads.
+    auto step = [&](const int s0, auto k, auto kn, const double *in_next, double *ow, int &lrel)
+                    __attribute__((always_inline)) {
+        const double *wcur = slot_ptr(k);
+        const double *wnext = slot_ptr(kn);
         // Input from below: lane l-1's outputs of the step before (same
         // columns), and for lane 0 the row under the band.
         double dl[C], vl[C];
@@ -377,74 +458,76 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, const d
             dl[c] = dpp_f64(bD[c], dout[c], 0);
             vl[c] = dpp_f64(bV[c], vout[c], 0);
         }
-        for_groups<R, C, 0>([&](auto G) __attribute__((always_inline)) {
+        static_for<0, NG>([&](auto G) __attribute__((always_inline)) {
             constexpr int g = decltype(G)::value;
             if constexpr (g + 1 < NG) {
-                chain_load<R, C, g + 1>(wcur, w[g + 1]);
+                chain_load<R, C, GP, g + 1>(wcur, w[g + 1]);
             } else {
-                if (s + 1 < nsteps) {
-                    wait_prod(s + 1);
-                    wait_pred(s + 1);
+                if (__builtin_expect(k >= lrel, 0)) {
+                    wait_for(s0 + k);
+                    lim = limit();
+                    lrel = lim - s0;
                 }
-                chain_load<R, C, 0>(wnext, w[0]);
-                load_in(s + 1);
+                chain_load<R, C, GP, 0>(wnext, w[0]);
+                load_in(in_next);
             }
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            chain_rows<R, C, g>(w[g], dbot, vbot, dl, vl);
-            chain_done<R, C, g>(dbot, vbot, dl, vl);
+            chain_rows<R, C, GP, g>(w[g], dbot, vbot, dl, vl);
+            chain_done<R, C, GP, g>(dbot, vbot, dl, vl);
         });
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             dout[c] = dl[c];
             vout[c] = vl[c];
+            ow[2 * c] = dout[c];
+            ow[2 * c + 1] = vout[c];
         }
-        if (s == s_term) {
-            // Terminal cell: its DFD value is the row's top; its VE-FD value
-            // is min(T, R), where R left row tl and, rows above tl being
-            // padding that passes VE-FD values through, is the lane's output.
-            if (lane == tlane) {
-                double topD = dbot[0], topV = vbot[0];
-#pragma unroll
-                for (int t = 1; t < R; ++t) {
-                    topD = (tl == t) ? dbot[t] : topD;
-                    topV = (tl == t) ? vbot[t] : topV;
-                }
-                const double resV = dmin(topV, vout[C - 1]);
-                a.result[0] = __builtin_sqrt(topD);
-                a.result[1] = topD;
-                a.result[2] = __builtin_sqrt(resV);
-                a.result[3] = resV;
-            }
-        }
-        // Step s's weights are all in registers: its ring slot is free.
-        if (lane == 0) lds_put(sy.cons, s + 1);
-        slot = nslot;
+        // The outputs are in the ring and step s0 + k's weights and input in
+        // registers: the step is finished.
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        vdone += 1;
+        lds_put(&sh.cons[lane], vdone);
+    };
 
-        // Row 64R(b+1) - 1 at columns C(s - 63) .. (lane 63's outputs) into
-        // the outgoing ring, for the courier to publish.
-        if (has_succ && s >= 63) {
-            const int col = C * (s - 63);
-            if (col + C > done_known + kRing) {
-                for (;;) {
-                    done_known = __builtin_amdgcn_readfirstlane(lds_get(sy.out_done));
-                    if (col + C <= done_known + kRing) break;
-                    __builtin_amdgcn_s_sleep(1);
-                }
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            }
-            if (lane == 63) {
-#pragma unroll
-                for (int c = 0; c < C; ++c) {
-                    const int r = (col + c) & (kRing - 1);
-                    outring[r] = dout[c];
-                    outring[kRing + r] = vout[c];
-                }
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                lds_put(sy.out_avail, col + C);
-            }
-        }
+    // Trips of KS steps, one per ring slot, with every LDS offset constant;
+    // bi = C s0 mod KB.
+    const int nfull = nsteps - nsteps % KS;
+    int s0 = 0, bi = 0;
+    for (; s0 < nfull; s0 += KS) {
+        const int bin = (bi + C * KS == KB) ? 0 : bi + C * KS;
+        double *ow = (lane == 63) ? sh.outring + 2 * bi : sh.odummy + 2 * lane;
+        int lrel = lim - s0;
+        static_for<0, KS>([&](auto K) __attribute__((always_inline)) {
+            constexpr int k = decltype(K)::value;
+            step(s0, K, std::integral_constant<int, (k + 1) % KS>{},
+                 sh.inring + 2 * (k + 1 < KS ? bi + C * (k + 1) : bin), ow + 2 * C * k, lrel);
+        });
+        bi = bin;
+    }
+    {
+        // The last nsteps mod KS (< KS) steps, from slots 0 .. KS - 2.
+        double *ow = (lane == 63) ? sh.outring + 2 * bi : sh.odummy + 2 * lane;
+        int lrel = lim - s0;
+        for (int k = 0; s0 + k < nsteps; ++k)
+            step(s0, k, k + 1, sh.inring + 2 * (bi + C * (k + 1)), ow + 2 * C * k, lrel);  // k + 1 < KS
     }
     __builtin_amdgcn_s_setprio(0);
+    if (has_term && lane == tlane) {
+        // Terminal cell: its DFD value is the row's top; its VE-FD value is
+        // min(T, R), where R left row tl and, rows above tl being padding that
+        // passes VE-FD values through, is the lane's output.
+        double topD = dbot[0], topV = vbot[0];
+#pragma unroll
+        for (int t = 1; t < R; ++t) {
+            topD = (tl == t) ? dbot[t] : topD;
+            topV = (tl == t) ? vbot[t] : topV;
+        }
+        const double resV = dmin(topV, vout[C - 1]);
+        a.result[0] = __builtin_sqrt(topD);
+        a.result[1] = topD;
+        a.result[2] = __builtin_sqrt(resV);
+        a.result[3] = resV;
+    }
     if (a.tlog && lane == 0) {
         a.tlog[4 * band + 2] = wall_clock64();
         a.tlog[4 * band + 3] = ((unsigned long long)__builtin_amdgcn_s_getreg((20 << 0) | (0 << 6) | (31 << 11)) << 32) |
@@ -453,68 +536,77 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, const d
 }
 
 // The band's global-memory hand-offs, kept off the chain: the courier copies
-// the predecessor's published boundary columns into the incoming ring and
-// publishes the chain's outgoing columns (scoped stores, s_waitcnt, flag), as
-// many columns per round as are ready, up to 64.  NK values per column
-// (bound sweep: DFD and VE-FD; decision: one); slots are [buffer][NK][cols].
-template <int NK = 2, int C = 1>
-__device__ inline void bounds_courier(const BoundsArgs &a, const int band, double *inring,
-                                      const double *outring, const BoundsSync &sy) {
+// the predecessor's published boundary columns into the incoming ring (the
+// first band's row -1 it writes itself: DFD 0 at column 0 and +big after it,
+// VE-FD +big) and publishes the chain's outgoing columns (scoped stores,
+// s_waitcnt, flag), as many columns per round as are ready, up to 64.  Slots
+// are [2 buffers][DFD, VE-FD][cols].
+template <int R, int NPW, int C>
+__device__ inline void bounds_courier(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C> &sh) {
+    constexpr int KB = BoundsShared<R, NPW, C>::KB;
     const int lane = (int)(threadIdx.x & 63);
-    const long long cols = a.cols;
-    const double *inD = a.slots + (long long)((band + 1) & 1) * NK * cols;
-    const double *inV = inD + (NK - 1) * cols;
-    double *outD = a.slots + (long long)(band & 1) * NK * cols;
-    double *outV = outD + (NK - 1) * cols;
-    long long in_next = band > 0 ? 0 : cols;
-    long long out_next = band + 1 < a.nbands ? 0 : cols;
+    const int cols = (int)a.cols;
+    const double big = (double)FLT_MAX;
+    const double *inD = a.slots + (long long)((band + 1) & 1) * 2 * cols;
+    const double *inV = inD + cols;
+    double *outD = a.slots + (long long)(band & 1) * 2 * cols;
+    double *outV = outD + cols;
+    const bool first = band == 0;
+    int in_next = 0;
+    int out_next = band + 1 < a.nbands ? 0 : cols;
     while (in_next < cols || out_next < cols) {
         bool moved = false;
+        // Steps the chain has finished: their incoming columns are consumed
+        // and their outputs are in the ring.
+        const int done = __builtin_amdgcn_readfirstlane(lds_get(&sh.cons[0]));
         if (in_next < cols) {
-            // Ring slots free: the chain has consumed the columns of the
-            // steps below `cons` (C per step).
-            const long long room =
-                (long long)C * __builtin_amdgcn_readfirstlane(lds_get(sy.cons)) + kRing - in_next;
+            const int room = C * done + KB - in_next;
             if (room > 0) {
-                const long long f = __builtin_amdgcn_readfirstlane(flag_load<SYNC_SCOPED>(&a.progress[band - 1]));
-                long long n = f - in_next;
+                const int f = first ? cols : __builtin_amdgcn_readfirstlane(flag_load<SYNC_SCOPED>(&a.progress[band - 1]));
+                int n = f - in_next;
                 n = n < room ? n : room;
                 n = n < 64 ? n : 64;
                 if (n > 0) {
                     __atomic_signal_fence(__ATOMIC_SEQ_CST);
                     if (lane < n) {
-                        const long long c = in_next + lane;
-                        const double vD = column_load<SYNC_SCOPED>(inD + c);
-                        inring[c % kRing] = vD;
-                        if (NK > 1) {
-                            const double vV = column_load<SYNC_SCOPED>(inV + c);
-                            inring[kRing + c % kRing] = vV;
+                        const int c = in_next + lane;
+                        double vD, vV;
+                        if (first) {
+                            vD = (c == 0) ? 0.0 : big;
+                            vV = big;
+                        } else {
+                            vD = column_load<SYNC_SCOPED>(inD + c);
+                            vV = column_load<SYNC_SCOPED>(inV + c);
                         }
+                        const int p = c % KB;
+                        sh.inring[2 * p] = vD;
+                        sh.inring[2 * p + 1] = vV;
                     }
                     __atomic_signal_fence(__ATOMIC_SEQ_CST);
                     __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the ring writes are done
                     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                    if (lane == 0) lds_put(sy.in_avail, (int)(in_next + n));
+                    if (lane == 0) lds_put(&sh.in_avail, in_next + n);
                     in_next += n;
                     moved = true;
                 }
             }
         }
         if (out_next < cols) {
-            const long long oa = __builtin_amdgcn_readfirstlane(lds_get(sy.out_avail));
-            long long n = oa - out_next;
+            // Lane 63 writes columns C (s - 63) .. at step s.
+            int n = C * (done - 63) - out_next;
             n = n < 64 ? n : 64;
             if (n > 0) {
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
                 if (lane < n) {
-                    const long long c = out_next + lane;
-                    column_store<SYNC_SCOPED>(outD + c, outring[c % kRing]);
-                    if (NK > 1) column_store<SYNC_SCOPED>(outV + c, outring[kRing + c % kRing]);
+                    const int c = out_next + lane;
+                    const int p = (c + 63 * C) % KB;
+                    column_store<SYNC_SCOPED>(outD + c, sh.outring[2 * p]);
+                    column_store<SYNC_SCOPED>(outV + c, sh.outring[2 * p + 1]);
                 }
                 __builtin_amdgcn_wave_barrier();
-                if (lane == 0) flag_store<SYNC_SCOPED>(&a.progress[band], (int)(out_next + n));
+                if (lane == 0) flag_store<SYNC_SCOPED>(&a.progress[band], out_next + n);
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                if (lane == 0) lds_put(sy.out_done, (int)(out_next + n));
+                if (lane == 0) lds_put(&sh.out_done, out_next + n);
                 out_next += n;
                 moved = true;
             }
@@ -523,18 +615,13 @@ __device__ inline void bounds_courier(const BoundsArgs &a, const int band, doubl
     }
 }
 
-// Columns of Q staged in LDS for the producers: 4C chunks of 64 columns,
-// component-major (component k of column j at k * qring_cols(C) + (j mod
-// qring_cols(C))).
-constexpr int qring_cols(int C) { return 256 * C; }
-
-template <int D, int R, int NPW, int KS, int C>
+// Producer wave pw: rows t = pw, pw + NPW, ... (< R) of every lane, NR of
+// them.
+template <int D, int R, int NPW, int KS, int C, int NR>
 __device__ inline void bounds_producer(const BoundsArgs &a, const int band, const int pw, double *wring,
                                        double *qring, int *prod, int *cons) {
     constexpr int QR = qring_cols(C);
-    // Rows t = pw, pw + NPW, ... (< R) of every lane.
-    constexpr int RP = (R + NPW - 1) / NPW;
-    const int nmine = (R - pw + NPW - 1) / NPW;
+    constexpr int RP = NR;
     const int lane = (int)(threadIdx.x & 63);
     const long long rows = a.rows;
     const int cols = (int)a.cols;
@@ -579,7 +666,7 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
     }
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     const double r2 = a.r2;
-    const int nsteps = cols / C + 63;
+    const int nsteps = band_steps<R, C>(a, band);
     int cons_seen = 0;
     int slot = 0;
     // Carried from the column before: q_{j-1}, and p_i - q_{j-1} of each row,
@@ -595,12 +682,24 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
 #pragma unroll
         for (int c = 0; c < D; ++c) xA[k][c] = 0.0;
 
+    // q of the wave's next column, loaded a column ahead (a wave's columns
+    // follow each other: j, j + 1, ...).  At j < 0 it reads ring slots not yet
+    // holding Q; those values feed only weights that the boundary rules
+    // overwrite.
+    double qp[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) qp[k] = qring[k * QR + ((C * (0 - lane)) & (QR - 1))];
+
     // Weights of column j for this wave's rows into w (one column of a slot).
     auto column = [&](const int j, const double(&qa)[D], double(&qb)[D], const double(&xp)[RP][D],
                       double(&xn)[RP][D], double *w, const bool edge) __attribute__((always_inline)) {
-        const int qi = j & (QR - 1);
 #pragma unroll
-        for (int k = 0; k < D; ++k) qb[k] = qring[k * QR + qi];
+        for (int k = 0; k < D; ++k) qb[k] = qp[k];
+        {
+            const int qi = (j + 1) & (QR - 1);
+#pragma unroll
+            for (int k = 0; k < D; ++k) qp[k] = qring[k * QR + qi];
+        }
         // Column terms of segment [q_{j-1}, q_j].
         double cab[D], cab2 = 0.0;
 #pragma unroll
@@ -610,7 +709,6 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         }
 #pragma unroll
         for (int k = 0; k < RP; ++k) {
-            if (RP * NPW != R && k == RP - 1 && k >= nmine) break;
             const int t = pw + k * NPW;
             double d = 0.0;
 #pragma unroll
@@ -632,7 +730,6 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
             asm volatile("");
 #pragma unroll
             for (int k = 0; k < RP; ++k) {
-                if (RP * NPW != R && k == RP - 1 && k >= nmine) break;
                 const int t = pw + k * NPW;
                 const long long i = i0 + t;
                 const bool pad = i >= rows;  // padding rows of the last band pass VE-FD values up unchanged
@@ -671,8 +768,8 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         }
     };
     auto step_end = [&](const int s) __attribute__((always_inline)) {
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this step's weights are in LDS
+        // A wave's LDS operations complete in order, so a reader that sees
+        // the count sees this step's weights.
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         if (lane == 0) lds_put(&prod[pw], s + 1);
         slot = (slot + 1 == KS) ? 0 : slot + 1;
@@ -709,18 +806,6 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
     }
 }
 
-// LDS of one workgroup of the fused sweep.
-template <int R, int NPW, int C = 1> struct BoundsShared {
-    static constexpr int KS = BoundsRing<R, NPW, C>::KS;
-    double wring[KS * C * R * 3 * 64];
-    double qring[3 * qring_cols(C)];
-    double inring[2 * kRing];
-    double outring[2 * kRing];
-    int sprod[NPW];
-    int swords[4];  // cons, in_avail, out_avail, out_done
-    int item;
-};
-
 // One band of one pair, by every wave of the workgroup.  The caller has
 // cleared the sync words and ends with a barrier before reusing the LDS.
 // C = 2 (two columns per step) requires n even.
@@ -728,24 +813,31 @@ template <int D, int R, int NPW, int C = 1>
 __device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C> &sh) {
     constexpr int KS = BoundsRing<R, NPW, C>::KS;
     const int wave = (int)(threadIdx.x >> 6);
-    const BoundsSync sy{sh.sprod, &sh.swords[0], &sh.swords[1], &sh.swords[2], &sh.swords[3]};
     // Waves of a workgroup are dealt to the CU's four SIMDs in turn, so wave 4
     // shares a SIMD with wave 0: that is the courier, which mostly sleeps, and
     // the chain's SIMD keeps its issue slots for the chain.
     constexpr int cw = (NPW + 2 > 4) ? 4 : NPW + 1;
-    if (wave == 0)
-        bounds_chain<R, KS, C>(a, band, sh.wring, sh.inring, sh.outring, sy, NPW);
-    else if (wave == cw)
-        bounds_courier<2, C>(a, band, sh.inring, sh.outring, sy);
-    else
-        bounds_producer<D, R, NPW, KS, C>(a, band, wave - 1 - (wave > cw ? 1 : 0), sh.wring, sh.qring, sh.sprod,
-                                          sy.cons);
+    constexpr int RP = (R + NPW - 1) / NPW;
+    if (wave == 0) {
+        bounds_chain<R, NPW, C>(a, band, sh);
+    } else if (wave == cw) {
+        bounds_courier<R, NPW, C>(a, band, sh);
+    } else {
+        const int pw = wave - 1 - (wave > cw ? 1 : 0);
+        if (RP * NPW == R || pw < R - (RP - 1) * NPW) {
+            bounds_producer<D, R, NPW, KS, C, RP>(a, band, pw, sh.wring, sh.qring, sh.sprod, &sh.cons[0]);
+        } else if constexpr (RP > 1) {
+            bounds_producer<D, R, NPW, KS, C, RP - 1>(a, band, pw, sh.wring, sh.qring, sh.sprod, &sh.cons[0]);
+        }
+    }
 }
 
 template <int R, int NPW, int C>
 __device__ inline void bounds_clear(BoundsShared<R, NPW, C> &sh) {
     for (int w = 0; w < NPW; ++w) sh.sprod[w] = 0;
-    for (int k = 0; k < 4; ++k) sh.swords[k] = 0;
+    sh.cons[0] = 0;
+    sh.in_avail = 0;
+    sh.out_done = 0;
 }
 
 template <int D, int R, int NPW, int C>

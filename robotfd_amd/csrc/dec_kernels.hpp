@@ -27,6 +27,80 @@
 
 namespace rfd {
 
+// Columns of the band-boundary rings in LDS (incoming and outgoing).
+constexpr int kDecRing = 128;
+
+// The band's global-memory hand-offs, kept off the chain: the courier copies
+// the predecessor's published boundary columns into the incoming ring and
+// publishes the chain's outgoing columns (scoped stores, s_waitcnt, flag), as
+// many columns per round as are ready, up to 64.  NK values per column (one
+// for the decision); slots are [buffer][NK][cols].
+template <int NK = 1, int C = 1>
+__device__ inline void dec_courier(const BoundsArgs &a, const int band, double *inring,
+                                      const double *outring, const BoundsSync &sy) {
+    const int lane = (int)(threadIdx.x & 63);
+    const long long cols = a.cols;
+    const double *inD = a.slots + (long long)((band + 1) & 1) * NK * cols;
+    const double *inV = inD + (NK - 1) * cols;
+    double *outD = a.slots + (long long)(band & 1) * NK * cols;
+    double *outV = outD + (NK - 1) * cols;
+    long long in_next = band > 0 ? 0 : cols;
+    long long out_next = band + 1 < a.nbands ? 0 : cols;
+    while (in_next < cols || out_next < cols) {
+        bool moved = false;
+        if (in_next < cols) {
+            // Ring slots free: the chain has consumed the columns of the
+            // steps below `cons` (C per step).
+            const long long room =
+                (long long)C * __builtin_amdgcn_readfirstlane(lds_get(sy.cons)) + kDecRing - in_next;
+            if (room > 0) {
+                const long long f = __builtin_amdgcn_readfirstlane(flag_load<SYNC_SCOPED>(&a.progress[band - 1]));
+                long long n = f - in_next;
+                n = n < room ? n : room;
+                n = n < 64 ? n : 64;
+                if (n > 0) {
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    if (lane < n) {
+                        const long long c = in_next + lane;
+                        const double vD = column_load<SYNC_SCOPED>(inD + c);
+                        inring[c % kDecRing] = vD;
+                        if (NK > 1) {
+                            const double vV = column_load<SYNC_SCOPED>(inV + c);
+                            inring[kDecRing + c % kDecRing] = vV;
+                        }
+                    }
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the ring writes are done
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    if (lane == 0) lds_put(sy.in_avail, (int)(in_next + n));
+                    in_next += n;
+                    moved = true;
+                }
+            }
+        }
+        if (out_next < cols) {
+            const long long oa = __builtin_amdgcn_readfirstlane(lds_get(sy.out_avail));
+            long long n = oa - out_next;
+            n = n < 64 ? n : 64;
+            if (n > 0) {
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                if (lane < n) {
+                    const long long c = out_next + lane;
+                    column_store<SYNC_SCOPED>(outD + c, outring[c % kDecRing]);
+                    if (NK > 1) column_store<SYNC_SCOPED>(outV + c, outring[kDecRing + c % kDecRing]);
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0) flag_store<SYNC_SCOPED>(&a.progress[band], (int)(out_next + n));
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                if (lane == 0) lds_put(sy.out_done, (int)(out_next + n));
+                out_next += n;
+                moved = true;
+            }
+        }
+        if (!moved) __builtin_amdgcn_s_sleep(2);
+    }
+}
+
 // Ring depth in steps, 2048 R bytes each (as BoundsRing).
 template <int R> struct DecRing {
     static constexpr int budget = (R <= 8) ? 40960 : 122880;
@@ -109,7 +183,7 @@ __device__ inline void dec_chain(const BoundsArgs &a, const int band, const doub
         }
         const double *w = wring + (long long)slot * (R * 4 * 64) + lane;
         double dl = shift_up(dout);
-        const double bD = has_pred ? inring[(int)((s < cols ? s : cols - 1) % kRing)] : -1.0;
+        const double bD = has_pred ? inring[(int)((s < cols ? s : cols - 1) % kDecRing)] : -1.0;
         if (lane == 0) dl = bD;
         if (j >= 0 && j < cols) {
             dec_cells<R>(w, bot, dl);
@@ -120,16 +194,16 @@ __device__ inline void dec_chain(const BoundsArgs &a, const int band, const doub
         slot = (slot + 1 == KS) ? 0 : slot + 1;
         if (has_succ && s >= 63) {
             const long long col = s - 63;
-            if (col >= done_known + kRing) {
+            if (col >= done_known + kDecRing) {
                 for (;;) {
                     done_known = __builtin_amdgcn_readfirstlane(lds_get(sy.out_done));
-                    if (col < done_known + kRing) break;
+                    if (col < done_known + kDecRing) break;
                     __builtin_amdgcn_s_sleep(1);
                 }
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
             }
             if (lane == 63) {
-                outring[(int)(col % kRing)] = dout;
+                outring[(int)(col % kDecRing)] = dout;
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
                 lds_put(sy.out_avail, (int)(col + 1));
             }
@@ -261,8 +335,8 @@ __device__ inline void dec_producer(const BoundsArgs &a, const int band, const i
 template <int R, int NPW> struct DecShared {
     static constexpr int KS = DecRing<R>::KS;
     double wring[KS * R * 4 * 64];
-    double inring[kRing];
-    double outring[kRing];
+    double inring[kDecRing];
+    double outring[kDecRing];
     int sprod[NPW];
     int swords[4];  // cons, in_avail, out_avail, out_done
     int item;
@@ -285,7 +359,7 @@ __device__ inline void dec_band(const BoundsArgs &a, const int band, DecShared<R
     if (wave == 0)
         dec_chain<D, R, KS>(a, band, sh.wring, sh.inring, sh.outring, sy, NPW);
     else if (wave == cw)
-        bounds_courier<1>(a, band, sh.inring, sh.outring, sy);
+        dec_courier<1>(a, band, sh.inring, sh.outring, sy);
     else
         dec_producer<D, R, NPW, KS>(a, band, wave - 1 - (wave > cw ? 1 : 0), sh.wring, sh.sprod, sy.cons);
 }
