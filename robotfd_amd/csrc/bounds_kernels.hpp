@@ -70,6 +70,10 @@ __device__ inline void lds_put(int *p, int v) { __hip_atomic_store(p, v, __ATOMI
 // component-major (component k of column j at k * qring_cols(C) + (j mod
 // qring_cols(C))).
 constexpr int qring_cols(int C) { return 256 * C; }
+// Components per Q-ring column: q (3), and with five or more producers also
+// the column's segment terms q_j - q_{j-1} (3) and |q_j - q_{j-1}|^2, staged
+// once per chunk instead of recomputed by every producer wave every step.
+constexpr int qring_comps(int NPW) { return NPW >= 5 ? 7 : 3; }
 
 // LDS plan of one workgroup.  The weight ring holds KS steps of 1536 R C bytes
 // (C columns per step), as many as fit (at most 16) beside the Q ring, the
@@ -85,7 +89,8 @@ template <int R, int NPW, int C = 1> struct BoundsRing {
         return ((min_cols + C * ks - 1) / (C * ks)) * (C * ks);
     }
     static constexpr int bytes(int ks) {
-        return ks * 1536 * R * C + 3 * qring_cols(C) * 8 + 2 * 2 * kb(ks) * 8 + 2 * (64 + C * ks) * 8 + 64 * 4 +
+        return ks * 1536 * R * C + qring_comps(NPW) * qring_cols(C) * 8 + 2 * 2 * kb(ks) * 8 + 2 * (64 + C * ks) * 8 +
+               64 * 4 +
                16 * 4 + 64;
     }
     static constexpr int cap = (NPW < 5) ? 163840 / 3 : 163840;
@@ -299,7 +304,7 @@ template <int R, int NPW, int C = 1> struct BoundsShared {
     alignas(16) double inring[2 * KB];
     alignas(16) double outring[2 * KB];
     alignas(16) double odummy[2 * (64 + C * KS)];
-    alignas(16) double qring[3 * qring_cols(C)];
+    alignas(16) double qring[qring_comps(NPW) * qring_cols(C)];
     alignas(16) double wring[KS * C * R * 3 * 64];
 };
 
@@ -654,15 +659,38 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
     // steps ahead of the chain), so no wave still reads the chunk 4C chunks
     // back, whose slots the stored chunk takes, and none reads a chunk before
     // its own store of it (a wave's LDS operations complete in order).
+    constexpr bool QT = qring_comps(NPW) == 7;
+    // Stores column c's entries: q_c, and with the segment terms q_c -
+    // q_{c-1} and its squared length (the operations of the column terms
+    // below, so the same values), q_{c-1} being `qm`.
+    auto put_column = [&](const int idx, const double(&qc)[D], const double(&qm)[D]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) qring[k * QR + idx] = qc[k];
+        if constexpr (QT) {
+            double c2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const double cab = qc[k] - qm[k];
+                c2 = __builtin_fma(cab, cab, c2);
+                qring[(3 + k) * QR + idx] = cab;
+            }
+            qring[6 * QR + idx] = c2;
+        }
+    };
     double qn[D];
     {
         const int c0 = lane < cols ? lane : cols - 1;
         const int c1 = 64 + lane < cols ? 64 + lane : cols - 1;
+        double q0[D], qm[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            qring[k * QR + lane] = Q[(long long)c0 * D + k];
+            q0[k] = Q[(long long)c0 * D + k];
             qn[k] = Q[(long long)c1 * D + k];
         }
+        // Column 0 has no q_{-1}; its right weights are boundary values.
+#pragma unroll
+        for (int k = 0; k < D; ++k) qm[k] = dpp_f64(q0[k], q0[k], 0);
+        put_column(lane, q0, qm);
     }
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     const double r2 = a.r2;
@@ -686,27 +714,37 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
     // follow each other: j, j + 1, ...).  At j < 0 it reads ring slots not yet
     // holding Q; those values feed only weights that the boundary rules
     // overwrite.
-    double qp[D];
+    double qp[D], cp[D], c2p = 0.0;
+    auto prefetch = [&](const int qi) __attribute__((always_inline)) {
 #pragma unroll
-    for (int k = 0; k < D; ++k) qp[k] = qring[k * QR + ((C * (0 - lane)) & (QR - 1))];
+        for (int k = 0; k < D; ++k) qp[k] = qring[k * QR + qi];
+        if constexpr (QT) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) cp[k] = qring[(3 + k) * QR + qi];
+            c2p = qring[6 * QR + qi];
+        }
+    };
+    prefetch((C * (0 - lane)) & (QR - 1));
 
     // Weights of column j for this wave's rows into w (one column of a slot).
     auto column = [&](const int j, const double(&qa)[D], double(&qb)[D], const double(&xp)[RP][D],
                       double(&xn)[RP][D], double *w, const bool edge) __attribute__((always_inline)) {
-#pragma unroll
-        for (int k = 0; k < D; ++k) qb[k] = qp[k];
-        {
-            const int qi = (j + 1) & (QR - 1);
-#pragma unroll
-            for (int k = 0; k < D; ++k) qp[k] = qring[k * QR + qi];
-        }
         // Column terms of segment [q_{j-1}, q_j].
         double cab[D], cab2 = 0.0;
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-            cab[k] = qb[k] - qa[k];
-            cab2 = __builtin_fma(cab[k], cab[k], cab2);
+        for (int k = 0; k < D; ++k) qb[k] = qp[k];
+        if constexpr (QT) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) cab[k] = cp[k];
+            cab2 = c2p;
+        } else {
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                cab[k] = qb[k] - qa[k];
+                cab2 = __builtin_fma(cab[k], cab[k], cab2);
+            }
         }
+        prefetch((j + 1) & (QR - 1));
 #pragma unroll
         for (int k = 0; k < RP; ++k) {
             const int t = pw + k * NPW;
@@ -752,11 +790,16 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
             const int c = C * s + 64 + lane;  // the chunk after the one holding column Cs
             int cn = c + 64;                  // and the chunk after that, stored 64 columns later
             cn = cn < cols ? cn : cols - 1;
+            // q_{c-1}: lane l-1's q, and for lane 0 the chunk before's last
+            // column, already in the ring.
+            double qm[D];
+            if constexpr (QT) {
 #pragma unroll
-            for (int k = 0; k < D; ++k) {
-                qring[k * QR + (c & (QR - 1))] = qn[k];
-                qn[k] = Q[(long long)cn * D + k];
+                for (int k = 0; k < D; ++k) qm[k] = dpp_f64(qring[k * QR + ((C * s + 63) & (QR - 1))], qn[k], 0);
             }
+            put_column(c & (QR - 1), qn, qm);
+#pragma unroll
+            for (int k = 0; k < D; ++k) qn[k] = Q[(long long)cn * D + k];
         }
         if (s >= cons_seen + KS) {
             for (;;) {
