@@ -785,7 +785,9 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
             }
         }
     };
-    auto step_begin = [&](const int s) __attribute__((always_inline)) {
+    // `ahead`: steps past s whose ring slots must be free too (the step pair
+    // of the one-column loop checks and publishes once).
+    auto step_begin = [&](const int s, const int ahead) __attribute__((always_inline)) {
         if (((C * s) & 63) == 0) {
             const int c = C * s + 64 + lane;  // the chunk after the one holding column Cs
             int cn = c + 64;                  // and the chunk after that, stored 64 columns later
@@ -801,10 +803,10 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
 #pragma unroll
             for (int k = 0; k < D; ++k) qn[k] = Q[(long long)cn * D + k];
         }
-        if (s >= cons_seen + KS) {
+        if (s + ahead >= cons_seen + KS) {
             for (;;) {
                 cons_seen = __builtin_amdgcn_readfirstlane(lds_get(cons));
-                if (s < cons_seen + KS) break;
+                if (s + ahead < cons_seen + KS) break;
                 __builtin_amdgcn_s_sleep(1);
             }
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -822,7 +824,7 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         // The carried values alternate between the A and B registers from
         // one column to the next instead of being copied.
         for (int s = 0; s < nsteps; ++s) {
-            step_begin(s);
+            step_begin(s, 0);
             double *w = wring + slot * (C * R * 3 * 64) + lane;
             const bool edge = s < 64 || edge_band;
             const int j = C * (s - lane);
@@ -832,20 +834,30 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         }
     } else {
         // Two steps per trip, the carried values alternating between the A
-        // and B registers instead of being copied.
+        // and B registers instead of being copied.  The pair waits for ring
+        // room and publishes once: the chain then sees the producers two
+        // steps at a time, which needs a ring of three steps or more (a
+        // chain at step c needs c + 2 steps written; producers short of
+        // that are at most c + 1 < c + KS - 1, so they are not waiting).
+        // Chunk stores fall on multiples of 64, the pair's first step.
+        static_assert(KS >= 3, "step pairs need a ring of three steps");
         auto one = [&](const int s, const double(&qa)[D], double(&qb)[D], const double(&xp)[RP][D],
-                       double(&xn)[RP][D]) __attribute__((always_inline)) {
-            step_begin(s);
+                       double(&xn)[RP][D], const bool first, const bool last) __attribute__((always_inline)) {
+            if (first) step_begin(s, last ? 0 : 1);
             double *w = wring + slot * (R * 3 * 64) + lane;
             column(s - lane, qa, qb, xp, xn, w, s < 64 || edge_band);
-            step_end(s);
+            if (last) {
+                step_end(s);
+            } else {
+                slot = (slot + 1 == KS) ? 0 : slot + 1;
+            }
         };
         int s = 0;
         for (; s + 1 < nsteps; s += 2) {
-            one(s, qA, qB, xA, xB);
-            one(s + 1, qB, qA, xB, xA);
+            one(s, qA, qB, xA, xB, true, false);
+            one(s + 1, qB, qA, xB, xA, false, true);
         }
-        if (s < nsteps) one(s, qA, qB, xA, xB);
+        if (s < nsteps) one(s, qA, qB, xA, xB, true, true);
     }
 }
 
