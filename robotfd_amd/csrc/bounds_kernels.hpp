@@ -90,8 +90,7 @@ template <int R, int NPW, int C = 1> struct BoundsRing {
     }
     static constexpr int bytes(int ks) {
         return ks * 1536 * R * C + qring_comps(NPW) * qring_cols(C) * 8 + 2 * 2 * kb(ks) * 8 + 2 * (64 + C * ks) * 8 +
-               64 * 4 +
-               16 * 4 + 64;
+               2 * 64 * 4 + 16 * 4 + 64;
     }
     static constexpr int cap = (NPW < 5) ? 163840 / 3 : 163840;
     static constexpr int pick() {
@@ -298,6 +297,7 @@ template <int R, int NPW, int C = 1> struct BoundsShared {
     // 16-bit offset.
     alignas(16) int sprod[16];  // steps each producer has written
     int cons[64];               // steps the chain has finished (one copy per lane; read [0])
+    int wfree[64];              // steps whose ring slot the chain has read (one copy per lane; read [0])
     int in_avail;               // incoming columns in the ring
     int out_done;               // outgoing columns the courier has published
     int item;
@@ -467,6 +467,13 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
             constexpr int g = decltype(G)::value;
             if constexpr (g + 1 < NG) {
                 chain_load<R, C, GP, g + 1>(wcur, w[g + 1]);
+                if constexpr (g + 2 == NG) {
+                    // The step's last loads from its slot are issued, and a
+                    // wave's LDS operations complete in order: producers may
+                    // overwrite the slot once they see this count.
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    lds_put(&sh.wfree[lane], vdone + 1);
+                }
             } else {
                 if (__builtin_expect(k >= lrel, 0)) {
                     wait_for(s0 + k);
@@ -880,9 +887,9 @@ __device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsSh
     } else {
         const int pw = wave - 1 - (wave > cw ? 1 : 0);
         if (RP * NPW == R || pw < R - (RP - 1) * NPW) {
-            bounds_producer<D, R, NPW, KS, C, RP>(a, band, pw, sh.wring, sh.qring, sh.sprod, &sh.cons[0]);
+            bounds_producer<D, R, NPW, KS, C, RP>(a, band, pw, sh.wring, sh.qring, sh.sprod, &sh.wfree[0]);
         } else if constexpr (RP > 1) {
-            bounds_producer<D, R, NPW, KS, C, RP - 1>(a, band, pw, sh.wring, sh.qring, sh.sprod, &sh.cons[0]);
+            bounds_producer<D, R, NPW, KS, C, RP - 1>(a, band, pw, sh.wring, sh.qring, sh.sprod, &sh.wfree[0]);
         }
     }
 }
@@ -891,6 +898,7 @@ template <int R, int NPW, int C>
 __device__ inline void bounds_clear(BoundsShared<R, NPW, C> &sh) {
     for (int w = 0; w < NPW; ++w) sh.sprod[w] = 0;
     sh.cons[0] = 0;
+    sh.wfree[0] = 0;
     sh.in_avail = 0;
     sh.out_done = 0;
 }
