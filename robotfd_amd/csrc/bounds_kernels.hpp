@@ -35,8 +35,10 @@
 //
 // Hand-offs.  Inside the workgroup: producer w publishes the count of steps it
 // has written (an LDS word stored after its weights: a wave's LDS operations
-// complete in order); the chain publishes the count of steps it has finished;
-// a producer writes step s only once the chain has finished step s - KS.
+// complete in order); the chain publishes the count of steps whose ring slot
+// it has read (once its last loads from the slot are issued) and the count of
+// steps it has finished; a producer writes step s only once the chain has read
+// step s - KS.
 // Between bands: as band_sweep (scoped atomics, in-order band queue), with two
 // values per column (DFD, VE-FD), but carried by a third role, the courier
 // wave, through two LDS rings, so that the chain never waits on global memory.
@@ -628,7 +630,7 @@ __device__ inline void bounds_courier(const BoundsArgs &a, const int band, Bound
 }
 
 // Producer wave pw: rows t = pw, pw + NPW, ... (< R) of every lane, NR of
-// them.
+// them.  `cons`: the chain's count of steps whose ring slot it has read.
 template <int D, int R, int NPW, int KS, int C, int NR>
 __device__ inline void bounds_producer(const BoundsArgs &a, const int band, const int pw, double *wring,
                                        double *qring, int *prod, int *cons) {
