@@ -26,8 +26,8 @@
 // Layout.  Band b holds rows 64Rb .. 64R(b+1)-1; lane l of the chain wave owns
 // rows 64Rb + Rl + t, t < R, and at step s evaluates column s - l for all of
 // them (the anti-diagonal of band_sweep).  Producer wave w computes, for every
-// lane and step, the weights of rows t = w, w + NPW, ... (< R) into slot s mod KS
-// of the ring: three doubles per cell (DFD distance^2, VE-FD right and top
+// lane and step, the weights of a block of consecutive rows t into slot s mod
+// KS of the ring: three doubles per cell (DFD distance^2, VE-FD right and top
 // weights), lane-contiguous.  Producers read q_{s-l} from a ring of Q columns
 // in LDS (each wave stores every 64-column chunk itself, one chunk ahead, from
 // a coalesced load issued a chunk before that), and carry q_{j-1} and the
@@ -629,24 +629,36 @@ __device__ inline void bounds_courier(const BoundsArgs &a, const int band, Bound
     }
 }
 
-// Producer wave pw: rows t = pw, pw + NPW, ... (< R) of every lane, NR of
-// them.  `cons`: the chain's count of steps whose ring slot it has read.
+// Rows of producer wave pw within a lane: a block of consecutive rows, RP of
+// them for the first R - (RP - 1) NPW waves and RP - 1 for the others, RP =
+// ceil(R / NPW).
+template <int R, int NPW> struct ProducerRows {
+    static constexpr int RP = (R + NPW - 1) / NPW;
+    static constexpr int NBIG = R - (RP - 1) * NPW;  // waves with RP rows
+    static constexpr int first(int pw) { return pw < NBIG ? pw * RP : NBIG * RP + (pw - NBIG) * (RP - 1); }
+};
+
+// Producer wave pw: rows t = first(pw) .. first(pw) + NR - 1 of every lane.
+// `cons`: the chain's count of steps whose ring slot it has read.
 template <int D, int R, int NPW, int KS, int C, int NR>
 __device__ inline void bounds_producer(const BoundsArgs &a, const int band, const int pw, double *wring,
                                        double *qring, int *prod, int *cons) {
     constexpr int QR = qring_cols(C);
     constexpr int RP = NR;
+    const int tfirst = ProducerRows<R, NPW>::first(pw);
     const int lane = (int)(threadIdx.x & 63);
     const long long rows = a.rows;
     const int cols = (int)a.cols;
     const double big = (double)FLT_MAX;
     const double *P = a.P, *Q = a.Q;
     const long long i0 = (long long)band * 64 * R + (long long)R * lane;
-    // Row terms: p_i, p_{i-1}, p_i - p_{i-1} and its squared length.
+    // Row terms: p_i, p_{i-1}, p_i - p_{i-1} and its squared length.  A row
+    // after the wave's first takes p_{i-1} from the row before it (both
+    // indices clamp to m - 1 alike in the padding).
     double pi[RP][D], pm[RP][D], ab[RP][D], ab2[RP];
 #pragma unroll
     for (int k = 0; k < RP; ++k) {
-        const long long i = i0 + pw + (long long)k * NPW;
+        const long long i = i0 + tfirst + k;
         const long long ic = i < rows ? i : rows - 1;
         long long im = i - 1 < rows ? i - 1 : rows - 1;
         im = im > 0 ? im : 0;
@@ -654,7 +666,7 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
 #pragma unroll
         for (int c = 0; c < D; ++c) {
             pi[k][c] = P[ic * D + c];
-            pm[k][c] = P[im * D + c];
+            pm[k][c] = (k == 0) ? P[im * D + c] : pi[k - 1][c];
             ab[k][c] = pi[k][c] - pm[k][c];
             acc = __builtin_fma(ab[k][c], ab[k][c], acc);
         }
@@ -756,7 +768,7 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         prefetch((j + 1) & (QR - 1));
 #pragma unroll
         for (int k = 0; k < RP; ++k) {
-            const int t = pw + k * NPW;
+            const int t = tfirst + k;
             double d = 0.0;
 #pragma unroll
             for (int c = 0; c < D; ++c) {
@@ -764,7 +776,18 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
                 d = __builtin_fma(xn[k][c], xn[k][c], d);
             }
             const double wr = seg_w_ap<D>(pi[k], qa, xp[k], cab, cab2);
-            const double wt = seg_w<D>(qb, pm[k], ab[k], ab2[k]);
+            double wt;
+            if (k == 0) {
+                wt = seg_w<D>(qb, pm[k], ab[k], ab2[k]);
+            } else {
+                // q_j - p_{i-1} is -(p_{i-1} - q_j), the row before's
+                // difference (round to nearest is symmetric; a zero's sign
+                // cannot change the projection's sum, which starts at +0).
+                double qp_[D];
+#pragma unroll
+                for (int c = 0; c < D; ++c) qp_[c] = -xn[k - 1][c];
+                wt = seg_w_ap<D>(qb, pm[k], qp_, ab[k], ab2[k]);
+            }
             w[(t * 3 + 0) * 64] = d;
             w[(t * 3 + 1) * 64] = wr;
             w[(t * 3 + 2) * 64] = wt;
@@ -777,7 +800,7 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
             asm volatile("");
 #pragma unroll
             for (int k = 0; k < RP; ++k) {
-                const int t = pw + k * NPW;
+                const int t = tfirst + k;
                 const long long i = i0 + t;
                 const bool pad = i >= rows;  // padding rows of the last band pass VE-FD values up unchanged
                 if (j < 0) {
