@@ -78,6 +78,22 @@ def test_fused_configs_bit_exact(contexts, cfg, dims):
     assert st["bounds_f64"]["launches"] == len(SHAPES) and "dfd" not in st
 
 
+@pytest.mark.parametrize("cfg", [42, 77, 107, 1310, 21310])
+def test_fused_random_shapes_many_bands(contexts, cfg):
+    """Random shapes up to 6000 rows (up to eight bands of 13 rows per lane):
+    the band hand-offs, the first band's row -1, and the last band stopping at
+    the step where its terminal row reaches column n - 1, for terminal rows in
+    any lane."""
+    ctx = contexts(RFD_BOUNDS=cfg)
+    rng = np.random.default_rng(cfg)
+    for _ in range(12):
+        m, n = int(rng.integers(2, 6000)), int(rng.integers(2, 2000))
+        dims = int(rng.integers(2, 4))
+        P, Q = walk(rng, m, dims), walk(rng, n, dims)
+        say("random", cfg, dims, (m, n))
+        check_pair(ctx, P, Q)
+
+
 @pytest.mark.parametrize("cfg", [42, 84, 136, 137, 20137, 21310])
 def test_fused_workgroups_take_several_bands(contexts, cfg):
     ctx = contexts(RFD_BOUNDS=cfg, RFD_MAX_WAVES=3)
