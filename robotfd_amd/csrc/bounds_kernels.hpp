@@ -369,8 +369,14 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
     // three by the counts last read, so a step checks one scalar; at `lim` the
     // chain reads all the counts (waiting while one falls short) and moves it.
     int avail = 0, in_known = 0, out_known = 0;
+    // The producers' counts as plain loads, which the compiler merges into
+    // wide LDS reads (an atomic load per word costs the chain 1% here, where
+    // it reaches its limit every few steps).  Each count is one 32-bit word
+    // stored by one producer, and LDS dword accesses are single-copy atomic,
+    // so each element reads the old count or the new one; the clobber makes
+    // every poll read the words again.
     auto read_prod = [&]() __attribute__((always_inline)) {
-        asm volatile("" ::: "memory");  // read the words again on every poll
+        asm volatile("" ::: "memory");
         int mn = sh.sprod[0];
 #pragma unroll
         for (int k = 1; k < NPW; ++k) mn = sh.sprod[k] < mn ? sh.sprod[k] : mn;
