@@ -391,7 +391,9 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
             in_known = __builtin_amdgcn_readfirstlane(lds_get(&sh.in_avail));
             out_known = __builtin_amdgcn_readfirstlane(lds_get(&sh.out_done));
             if (avail >= np && in_known >= ni && (!has_succ || out_known >= no)) break;
-            __builtin_amdgcn_s_sleep(1);
+            // A longer nap: a chain that polls fast takes issue slots and LDS
+            // reads from the producers it waits for.
+            __builtin_amdgcn_s_sleep(8);
         }
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
     };
