@@ -122,53 +122,50 @@ __device__ inline double dpp_f64(double old, double src, int ctrl_shl) {
                                             (unsigned long long)(unsigned)lo));
 }
 
-// Squared distance of x to segment [a, b] given ab = b - a and |ab|^2; the
-// operation order of point_seg2 (.cu:375-394), with the segment terms hoisted.
-// The clamped parameter t = min(1, max(0, apab / ab2)) is 0 when apab <= 0
-// (or NaN) and 1 when apab >= ab2, whatever the quotient rounds to, so the
-// correctly rounded divide runs only for projections strictly inside the
-// segment: about 0.2% of point-segment pairs on long random walks, so most
-// waves skip it.
+// A cell's two point-segment weights: the squared distances of x1 to segment
+// [a1, a1 + b1] and of x2 to [a2, a2 + b2], given ap = x - a and l = |b|^2;
+// the operation order of point_seg2 (.cu:375-394), with the segment terms
+// hoisted.  The clamped parameter t = min(1, max(0, (ap . b) / l)) is 0 when
+// ap . b <= 0 (or NaN) and 1 when ap . b >= l, whatever the quotient rounds
+// to, so the correctly rounded divide runs only for projections strictly
+// inside the segment: about 0.2% of point-segment pairs on long random walks.
+// Both divides sit behind one branch: the branch, its masks and compares are
+// what a projection costs when it takes no divide, so sharing them is what
+// saves time (one branch per weight costs 2% at 200k x 200k).
 template <int D>
-__device__ inline double seg_w(const double *x, const double *a, const double *ab, double ab2) {
-    double apab = 0.0;
+__device__ inline void seg_w_pair(const double *x1, const double *a1, const double *ap1, const double *b1,
+                                  const double l1, const double *x2, const double *a2, const double *ap2,
+                                  const double *b2, const double l2, double &w1, double &w2) {
+    double p1 = 0.0, p2 = 0.0;
 #pragma unroll
-    for (int k = 0; k < D; ++k) apab = __builtin_fma(x[k] - a[k], ab[k], apab);
-    double t = (apab > 0.0) ? 1.0 : 0.0;
-    if (apab > 0.0 && apab < ab2) {
-        asm volatile("");  // keep the divide in a branch, not a select
-        t = apab / ab2;
-        t = __builtin_fmin(1.0, __builtin_fmax(0.0, t));
+    for (int k = 0; k < D; ++k) p1 = __builtin_fma(ap1[k], b1[k], p1);
+#pragma unroll
+    for (int k = 0; k < D; ++k) p2 = __builtin_fma(ap2[k], b2[k], p2);
+    double t1 = (p1 > 0.0) ? 1.0 : 0.0, t2 = (p2 > 0.0) ? 1.0 : 0.0;
+    const bool in1 = p1 > 0.0 && p1 < l1, in2 = p2 > 0.0 && p2 < l2;
+    if (in1 || in2) {
+        asm volatile("");  // one branch for both, not selects
+        if (in1) {
+            asm volatile("");
+            t1 = __builtin_fmin(1.0, __builtin_fmax(0.0, p1 / l1));
+        }
+        if (in2) {
+            asm volatile("");
+            t2 = __builtin_fmin(1.0, __builtin_fmax(0.0, p2 / l2));
+        }
     }
-    double w = 0.0;
+    w1 = 0.0;
+    w2 = 0.0;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-        const double d = x[k] - __builtin_fma(t, ab[k], a[k]);
-        w = __builtin_fma(d, d, w);
+        const double e = x1[k] - __builtin_fma(t1, b1[k], a1[k]);
+        w1 = __builtin_fma(e, e, w1);
     }
-    return w;
-}
-
-// seg_w with the difference x - a given (ap = x - a, as seg_w forms it).
-template <int D>
-__device__ inline double seg_w_ap(const double *x, const double *a, const double *ap, const double *ab,
-                                  double ab2) {
-    double apab = 0.0;
-#pragma unroll
-    for (int k = 0; k < D; ++k) apab = __builtin_fma(ap[k], ab[k], apab);
-    double t = (apab > 0.0) ? 1.0 : 0.0;
-    if (apab > 0.0 && apab < ab2) {
-        asm volatile("");  // keep the divide in a branch, not a select
-        t = apab / ab2;
-        t = __builtin_fmin(1.0, __builtin_fmax(0.0, t));
-    }
-    double w = 0.0;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-        const double d = x[k] - __builtin_fma(t, ab[k], a[k]);
-        w = __builtin_fma(d, d, w);
+        const double e = x2[k] - __builtin_fma(t2, b2[k], a2[k]);
+        w2 = __builtin_fma(e, e, w2);
     }
-    return w;
 }
 
 // min / max of doubles that are never NaN.  Where the translation unit is
@@ -783,19 +780,15 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
                 xn[k][c] = pi[k][c] - qb[c];
                 d = __builtin_fma(xn[k][c], xn[k][c], d);
             }
-            const double wr = seg_w_ap<D>(pi[k], qa, xp[k], cab, cab2);
-            double wt;
-            if (k == 0) {
-                wt = seg_w<D>(qb, pm[k], ab[k], ab2[k]);
-            } else {
-                // q_j - p_{i-1} is -(p_{i-1} - q_j), the row before's
-                // difference (round to nearest is symmetric; a zero's sign
-                // cannot change the projection's sum, which starts at +0).
-                double qp_[D];
+            // The top weight's q_j - p_{i-1}: for a row after the wave's first,
+            // -(p_{i-1} - q_j), the row before's difference (round to nearest
+            // is symmetric; a zero's sign cannot change the projection's sum,
+            // which starts at +0).
+            double dt[D];
 #pragma unroll
-                for (int c = 0; c < D; ++c) qp_[c] = -xn[k - 1][c];
-                wt = seg_w_ap<D>(qb, pm[k], qp_, ab[k], ab2[k]);
-            }
+            for (int c = 0; c < D; ++c) dt[c] = (k == 0) ? qb[c] - pm[k][c] : -xn[k - 1][c];
+            double wr, wt;
+            seg_w_pair<D>(pi[k], qa, xp[k], cab, cab2, qb, pm[k], dt, ab[k], ab2[k], wr, wt);
             w[(t * 3 + 0) * 64] = d;
             w[(t * 3 + 1) * 64] = wr;
             w[(t * 3 + 2) * 64] = wt;
