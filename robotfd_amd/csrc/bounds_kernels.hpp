@@ -873,10 +873,18 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         // chain at step c needs c + 2 steps written; producers short of
         // that are at most c + 1 < c + KS - 1, so they are not waiting).
         // Chunk stores fall on multiples of 64, the pair's first step.
+        // Configurations with five or more producers and at most 7 rows per
+        // lane (one row per producer, registers to spare) take four steps per
+        // wait and count, which needs a ring of five (the same argument with
+        // c + 1 < c + KS - 3): 57.5 -> 54.8 ms at 100k x 120k with 7 rows per
+        // lane, where 13 rows per lane (no registers to spare) took 147 ms
+        // and 4 rows with 2 producers 16% longer.
         static_assert(KS >= 3, "step pairs need a ring of three steps");
+        constexpr bool QUADS = KS >= 5 && NPW >= 5 && R <= 7;
         auto one = [&](const int s, const double(&qa)[D], double(&qb)[D], const double(&xp)[RP][D],
-                       double(&xn)[RP][D], const bool first, const bool last) __attribute__((always_inline)) {
-            if (first) step_begin(s, last ? 0 : 1);
+                       double(&xn)[RP][D], const bool first, const bool last, const int ahead)
+                       __attribute__((always_inline)) {
+            if (first) step_begin(s, last ? 0 : ahead);
             double *w = wring + slot * (R * 3 * 64) + lane;
             column(s - lane, qa, qb, xp, xn, w, s < 64 || edge_band);
             if (last) {
@@ -886,11 +894,19 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
             }
         };
         int s = 0;
-        for (; s + 1 < nsteps; s += 2) {
-            one(s, qA, qB, xA, xB, true, false);
-            one(s + 1, qB, qA, xB, xA, false, true);
+        if constexpr (QUADS) {
+            for (; s + 3 < nsteps; s += 4) {
+                one(s, qA, qB, xA, xB, true, false, 3);
+                one(s + 1, qB, qA, xB, xA, false, false, 0);
+                one(s + 2, qA, qB, xA, xB, false, false, 0);
+                one(s + 3, qB, qA, xB, xA, false, true, 0);
+            }
         }
-        if (s < nsteps) one(s, qA, qB, xA, xB, true, true);
+        for (; s + 1 < nsteps; s += 2) {
+            one(s, qA, qB, xA, xB, true, false, 1);
+            one(s + 1, qB, qA, xB, xA, false, true, 0);
+        }
+        if (s < nsteps) one(s, qA, qB, xA, xB, true, true, 0);
     }
 }
 
