@@ -873,14 +873,15 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         // chain at step c needs c + 2 steps written; producers short of
         // that are at most c + 1 < c + KS - 1, so they are not waiting).
         // Chunk stores fall on multiples of 64, the pair's first step.
-        // Configurations with five or more producers and at most 7 rows per
-        // lane (one row per producer, registers to spare) take four steps per
-        // wait and count, which needs a ring of five (the same argument with
-        // c + 1 < c + KS - 3): 57.5 -> 54.8 ms at 100k x 120k with 7 rows per
-        // lane, where 13 rows per lane (no registers to spare) took 147 ms
-        // and 4 rows with 2 producers 16% longer.
+        // Configurations with a deep ring (12 steps or more) and five or more
+        // producers take four steps per wait and count instead: free of
+        // deadlock from a ring of five (the same argument with c + 1 <
+        // c + KS - 3), and worth it only where the ring still decouples the
+        // roles after the coarser counts: 57.5 -> 54.8 ms at 100k x 120k (7
+        // rows per lane, a ring of 13), where rings of 7 took 147 ms (13 rows
+        // per lane, as a ring of 4 does with pairs) and 16% longer (4 rows).
         static_assert(KS >= 3, "step pairs need a ring of three steps");
-        constexpr bool QUADS = KS >= 5 && NPW >= 5 && R <= 7;
+        constexpr bool QUADS = KS >= 12 && NPW >= 5;
         auto one = [&](const int s, const double(&qa)[D], double(&qb)[D], const double(&xp)[RP][D],
                        double(&xn)[RP][D], const bool first, const bool last, const int ahead)
                        __attribute__((always_inline)) {
