@@ -10,7 +10,12 @@ namespace rfd {
 // Fused bound round: workgroups take (pair, band) items in order and run the
 // band with the single-pair fused code; a pair's DFD lands in its node-0
 // result, its VE-FD in node 1's, and the band hand-off uses nodes 0 and 1's
-// boundary columns (4n values).
+// boundary columns (4n values).  The items come from the batch's fixed list
+// when it has one (every pair takes part in this round), else from the plan.
+// Pairs of at most two dimensions run the two-dimensional producers over the
+// stored 3D points: their third coordinate is 0, and each term it would add is
+// fma(0, 0, acc) = acc, so the results are the same bits for a third less
+// point arithmetic.
 template <int R, int NPW>
 __global__ __launch_bounds__(64 * (NPW + 2)) void batch_bounds(BatchArgs a) {
     __shared__ BoundsShared<R, NPW> sh;
@@ -23,7 +28,7 @@ __global__ __launch_bounds__(64 * (NPW + 2)) void batch_bounds(BatchArgs a) {
         __syncthreads();
         const int idx = sh.item;
         if (idx >= total) break;
-        const int2 it = a.items[idx];
+        const int2 it = a.bitems ? a.bitems[idx] : a.items[idx];
         const int k = it.x / kSpec;
         const BatchPair p = a.pairs[k];
         BoundsArgs b;
@@ -38,7 +43,11 @@ __global__ __launch_bounds__(64 * (NPW + 2)) void batch_bounds(BatchArgs a) {
         b.nbands = (int)((p.m + 64LL * R - 1) / (64LL * R));
         b.r2 = p.endpoint_lb * p.endpoint_lb;
         b.tlog = nullptr;
-        bounds_band<3, R, NPW>(b, it.y, sh);
+        if (p.dims <= 2) {
+            bounds_band<2, R, NPW, 1, 3>(b, it.y, sh);
+        } else {
+            bounds_band<3, R, NPW>(b, it.y, sh);
+        }
         __syncthreads();
     }
 }

@@ -40,6 +40,7 @@ struct BatchPair {
     double endpoint_lb;      // frechet.cpp:30, computed on the host
     int big;                 // m*n > 10000 (.cu:1343, in 64 bits)
     int dec_form;            // decision interval form of this pair: DEC_FUSED or DEC_TWO_PHASE
+    int dims;                // dimensions of the pair's points (stored 3D either way)
 };
 
 struct BatchState {
@@ -65,6 +66,7 @@ struct BatchArgs {
     double *slots;
     int *progress;
     int2 *items;             // (pair, band) work list of the current round
+    const int2 *bitems;      // the fused bound round's work list, fixed per batch (or null)
     int *count;              // [0] items, [1] queue, [2] speculation depth of the round
     unsigned long long *evals;
     unsigned long long *t0;  // wall clock at batch start

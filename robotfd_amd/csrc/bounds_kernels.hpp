@@ -644,8 +644,10 @@ template <int R, int NPW> struct ProducerRows {
 };
 
 // Producer wave pw: rows t = first(pw) .. first(pw) + NR - 1 of every lane.
-// `cons`: the chain's count of steps whose ring slot it has read.
-template <int D, int R, int NPW, int KS, int C, int NR>
+// `cons`: the chain's count of steps whose ring slot it has read.  Points are
+// DS doubles apart, of which the first D are read (the batch stores every
+// curve 3D: its 2D pairs run D = 2 over DS = 3, whose extra coordinate is 0).
+template <int D, int R, int NPW, int KS, int C, int NR, int DS = D>
 __device__ inline void bounds_producer(const BoundsArgs &a, const int band, const int pw, double *wring,
                                        double *qring, int *prod, int *cons) {
     constexpr int QR = qring_cols(C);
@@ -670,8 +672,8 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         double acc = 0.0;
 #pragma unroll
         for (int c = 0; c < D; ++c) {
-            pi[k][c] = P[ic * D + c];
-            pm[k][c] = (k == 0) ? P[im * D + c] : pi[k - 1][c];
+            pi[k][c] = P[ic * DS + c];
+            pm[k][c] = (k == 0) ? P[im * DS + c] : pi[k - 1][c];
             ab[k][c] = pi[k][c] - pm[k][c];
             acc = __builtin_fma(ab[k][c], ab[k][c], acc);
         }
@@ -710,8 +712,8 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         double q0[D], qm[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            q0[k] = Q[(long long)c0 * D + k];
-            qn[k] = Q[(long long)c1 * D + k];
+            q0[k] = Q[(long long)c0 * DS + k];
+            qn[k] = Q[(long long)c1 * DS + k];
         }
         // Column 0 has no q_{-1}; its right weights are boundary values.
 #pragma unroll
@@ -834,7 +836,7 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
             }
             put_column(c & (QR - 1), qn, qm);
 #pragma unroll
-            for (int k = 0; k < D; ++k) qn[k] = Q[(long long)cn * D + k];
+            for (int k = 0; k < D; ++k) qn[k] = Q[(long long)cn * DS + k];
         }
         if (s + ahead >= cons_seen + KS) {
             for (;;) {
@@ -913,8 +915,8 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
 
 // One band of one pair, by every wave of the workgroup.  The caller has
 // cleared the sync words and ends with a barrier before reusing the LDS.
-// C = 2 (two columns per step) requires n even.
-template <int D, int R, int NPW, int C = 1>
+// C = 2 (two columns per step) requires n even.  DS: the points' stride.
+template <int D, int R, int NPW, int C = 1, int DS = D>
 __device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C> &sh) {
     constexpr int KS = BoundsRing<R, NPW, C>::KS;
     const int wave = (int)(threadIdx.x >> 6);
@@ -930,9 +932,9 @@ __device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsSh
     } else {
         const int pw = wave - 1 - (wave > cw ? 1 : 0);
         if (RP * NPW == R || pw < R - (RP - 1) * NPW) {
-            bounds_producer<D, R, NPW, KS, C, RP>(a, band, pw, sh.wring, sh.qring, sh.sprod, &sh.wfree[0]);
+            bounds_producer<D, R, NPW, KS, C, RP, DS>(a, band, pw, sh.wring, sh.qring, sh.sprod, &sh.wfree[0]);
         } else if constexpr (RP > 1) {
-            bounds_producer<D, R, NPW, KS, C, RP - 1>(a, band, pw, sh.wring, sh.qring, sh.sprod, &sh.wfree[0]);
+            bounds_producer<D, R, NPW, KS, C, RP - 1, DS>(a, band, pw, sh.wring, sh.qring, sh.sprod, &sh.wfree[0]);
         }
     }
 }

@@ -62,6 +62,7 @@ struct rfd_batch {
     double *pts = nullptr, *qseg = nullptr, *slots = nullptr;
     int *progress = nullptr;
     int2 *items = nullptr;
+    int2 *bitems = nullptr;    // the fused bound round's fixed work list
     int *count = nullptr;
     unsigned long long *evals = nullptr, *t0 = nullptr;
     float *feat = nullptr;     // [npairs x 386] FP32 features of the last bounds
@@ -109,6 +110,7 @@ rfd::BatchArgs make_args(rfd_batch *b, int rpl, const rfd_opts &o, int prune) {
     a.slots = b->slots;
     a.progress = b->progress;
     a.items = b->items;
+    a.bitems = b->bitems;
     a.count = b->count;
     a.evals = b->evals;
     a.t0 = b->t0;
@@ -262,6 +264,7 @@ rfd_status rfd_batch_destroy(rfd_batch *b) {
     (void)hipFree(b->slots);
     (void)hipFree(b->progress);
     (void)hipFree(b->items);
+    (void)hipFree(b->bitems);
     (void)hipFree(b->count);
     (void)hipFree(b->evals);
     (void)hipFree(b->t0);
@@ -313,6 +316,7 @@ rfd_status rfd_batch_create(rfd_ctx *ctx, int64_t npairs, const double *const *P
         p.endpoint_lb = rfd_host::endpoint_bound(P[k], m[k], Q[k], n[k], dims[k]);
         p.big = (m[k] * n[k] > 10000) ? 1 : 0;
         p.dec_form = rfd::DEC_FUSED;
+        p.dims = dims[k];
         cells_total += (double)m[k] * (double)n[k];
     }
     b->total_items = bofs;
@@ -322,6 +326,32 @@ rfd_status rfd_batch_create(rfd_ctx *ctx, int64_t npairs, const double *const *P
     std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) {
         return (double)m[x] * n[x] > (double)m[y] * n[y];
     });
+    // The fused bound round's work list.  Every pair takes part, so the list is
+    // fixed: pair-major in `order` (the plan's order), then spread so that a
+    // pair's band b sits kBandSpread places after its band b - 1.  Listed back
+    // to back, band b would be taken while band b - 1 is a few steps in, and
+    // its workgroup would wait ~64 steps per band of lag for its input rows:
+    // the wait of a tall pair's last band (25 bands of 832 rows) exceeds a
+    // narrow pair's whole band.  Spread by about one pick per workgroup, band
+    // b - 1 is a band's length ahead.  Keys are increasing within a pair, so
+    // every band still follows its predecessor in the list (no deadlock).
+    std::vector<int2> hbitems;
+    if (ctx->batch_bounds) {
+        constexpr long long kBandSpread = 256;
+        const int br = ctx->batch_bounds >= 1000 ? ctx->batch_bounds / 100 : ctx->batch_bounds / 10;
+        std::vector<std::pair<long long, int2>> keyed;
+        long long pos = 0;
+        for (int k : b->order) {
+            const long long nb = (m[k] + 64LL * br - 1) / (64LL * br);
+            for (long long bb = 0; bb < nb; ++bb)
+                keyed.push_back({pos + kBandSpread * bb, make_int2(k * rfd::kSpec, (int)bb)});
+            pos += nb;
+        }
+        std::stable_sort(keyed.begin(), keyed.end(),
+                         [](const auto &x, const auto &y) { return x.first < y.first; });
+        hbitems.reserve(keyed.size());
+        for (const auto &kv : keyed) hbitems.push_back(kv.second);
+    }
     char sig[128];
     std::snprintf(sig, sizeof(sig), "n%lld_c%.0f", (long long)npairs, std::log2(cells_total + 1) * 4);
     b->signature = sig;
@@ -349,6 +379,11 @@ rfd_status rfd_batch_create(rfd_ctx *ctx, int64_t npairs, const double *const *P
     if ((e = hipMalloc(&b->slots, sizeof(double) * slofs)) != hipSuccess) return fail(e, "hipMalloc(slots)");
     if ((e = hipMalloc(&b->progress, sizeof(int) * bofs)) != hipSuccess) return fail(e, "hipMalloc(progress)");
     if ((e = hipMalloc(&b->items, sizeof(int2) * bofs)) != hipSuccess) return fail(e, "hipMalloc(items)");
+    if (!hbitems.empty()) {
+        if ((e = hipMalloc(&b->bitems, sizeof(int2) * hbitems.size())) != hipSuccess) return fail(e, "hipMalloc(bound items)");
+        if ((e = hipMemcpy(b->bitems, hbitems.data(), sizeof(int2) * hbitems.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e, "hipMemcpy(bound items)");
+    }
     if ((e = hipMalloc(&b->count, sizeof(int) * 2)) != hipSuccess) return fail(e, "hipMalloc(count)");
     if ((e = hipMalloc(&b->evals, sizeof(unsigned long long))) != hipSuccess) return fail(e, "hipMalloc(evals)");
     if ((e = hipMalloc(&b->t0, sizeof(unsigned long long))) != hipSuccess) return fail(e, "hipMalloc(t0)");

@@ -130,3 +130,35 @@ def test_batch_autotuner_picks_a_valid_height(batch_and_ref):
     assert best in (1, 2, 3, 4)
     assert all(t > 0 for t in ms)
     assert ms[best - 1] == min(ms)
+
+
+def test_batch_bound_round_tall_pairs():
+    """The fused bound round on pairs of many bands (832 rows each at 13 rows
+    per lane) and few columns, 2D and 3D, whose bands the round's work list
+    spreads apart: bit-exact bounds and distances against the oracle."""
+    rng = np.random.default_rng(11)
+    pairs = []
+    for k in range(10):
+        m = int(rng.integers(1500, 5200))
+        n = int(rng.integers(40, 400))
+        d = 2 + (k % 2)
+        P = np.cumsum(rng.standard_normal((m, d)), axis=0)
+        Q = np.cumsum(rng.standard_normal((n, d)), axis=0)
+        if k >= 8:
+            P, Q = Q, P  # wide pairs too
+        pairs.append((P, Q))
+    refs = [O.distance(P, Q, stop_mode=1, eps=1e-9) for P, Q in pairs]
+    ctx = ctx_with(RFD_BATCH_BOUNDS=1310)
+    try:
+        with Batch(pairs, ctx=ctx) as b:
+            print("batch tall pairs", [(len(P), len(Q), P.shape[1]) for P, Q in pairs], flush=True)
+            out = b.run(stop="rel", eps=1e-9, rows_per_lane=2, spec_depth=1)
+            again = b.run(stop="rel", eps=1e-9, rows_per_lane=2, spec_depth=0)
+        for k, ref in enumerate(refs):
+            assert out["dfd"][k] == ref["dfd"], k
+            assert out["vefd"][k] == ref["vefd"], k
+            assert out["iterations"][k] == ref["iterations"], k
+            assert out["distance"][k] == ref["distance"], k
+            assert again["distance"][k] == ref["distance"], k
+    finally:
+        ctx.close()
