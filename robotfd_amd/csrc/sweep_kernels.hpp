@@ -8,7 +8,7 @@
 // inputs are its own previous cell (register) and lane l-1's previous output.
 //
 // Bands are taken from an in-order queue (one atomic per band).  Band b reads
-// row 64b-1's outputs from a column buffer that band b-1 fills in chunks of 64
+// row 64b-1's outputs from a column buffer that band b-1 fills in chunks of 16
 // columns; band b-1 publishes the count of finished columns with an agent-scope
 // release store, band b polls it with acquire loads.  Because the queue hands
 // out bands in order, every band a wave waits on is already held by a running
@@ -374,63 +374,72 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
 
     for (long long s = 0; s < nsteps; ++s) {
         const long long j = s - lane;
-        if ((s & 63) == 0) {
-            // Stage chunk k+1 (in registers) into LDS.
+        if ((s & 15) == 0) {
             const long long kc = s >> 6;
-            const int slot = (int)((kc + 1) & 3);
+            if ((s & 63) == 0) {
+                // Stage chunk k+1 (in registers) into LDS.
+                const int slot = (int)((kc + 1) & 3);
 #pragma unroll
-            for (int k = 0; k < D; ++k) qring[slot][k][lane] = pre[k];
-            if (NS > 0) {
-                sring[slot][0][lane] = pres[0];
-                sring[slot][NS - 1][lane] = pres[NS - 1];
+                for (int k = 0; k < D; ++k) qring[slot][k][lane] = pre[k];
+                if (NS > 0) {
+                    sring[slot][0][lane] = pres[0];
+                    sring[slot][NS - 1][lane] = pres[NS - 1];
+                }
             }
-            // Boundary values of this chunk; loaded before the prefetch below so
-            // that waiting for them does not wait for the prefetch.
+            // Boundary values of columns s .. s+15, into lanes (s mod 64) ..
+            // (s mod 64) + 15 of `bin` (lane l holds column 64 floor(s / 64) + l),
+            // 16 columns at a time so that a band trails its predecessor by
+            // about 64 + 16 steps, not 64 + 64.  Every lane loads (the other
+            // lanes' values are consumed or not yet needed, and are reloaded
+            // before use), clamped, so the load count is fixed and the wait for
+            // `bin` does not cover the prefetch issued after it.
+            const bool in_slice = (lane & 48) == (int)(s & 48);
             if (has_pred && s < cols) {
-                const long long need = (s + 64 < cols) ? s + 64 : cols;
+                const long long need = (s + 16 < cols) ? s + 16 : cols;
                 while (flag_load<SYNC>(&a.progress[band - 1]) < (int)need)
                     __builtin_amdgcn_s_sleep(1);
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                // Clamped (not predicated) so the load count is fixed and the
-                // wait for `bin` does not cover the prefetch issued after it.
-                const long long c = (s + lane < cols) ? s + lane : cols - 1;
+                const long long c = ((s & ~63LL) + lane < cols) ? (s & ~63LL) + lane : cols - 1;
                 bin = column_load<SYNC>(sin + c);
                 if (tlog && lane == 0 && s == 0) tlog[4 * band + 1] = wall_clock64();
             }
-            // Dead chunk: every lane's state and in-flight value is dead and so is
-            // the boundary input for columns s..s+63.  Then every cell of steps
-            // s..s+63 is dead; skip them, publishing dead outputs for columns
-            // s-63..s (lane 63's outputs of those steps).
+            // Dead block: every lane's state and in-flight value is dead and so
+            // is the boundary input for columns s..s+15.  Then every cell of
+            // steps s..s+15 is dead; skip them, publishing dead outputs for
+            // columns s-63..s-48 (lane 63's outputs of those steps).
             bool skip = false;
             if (OP != OP_DFD && s >= 64 && s + 128 < cols) {
                 bool mine = is_dead<OP>(out, thr);
 #pragma unroll
                 for (int t = 0; t < R; ++t) mine = mine && is_dead<OP>(bot[t], thr);
-                const T edge = has_pred ? bin : dead_out;
+                const T edge = (has_pred && in_slice) ? bin : dead_out;
                 mine = mine && is_dead<OP>(edge, thr);
                 skip = __builtin_amdgcn_read_exec() == __ballot(mine);
             }
-            // Fetch chunk k+2 into registers; it is written to LDS 64 steps later.
-            long long pn = (kc + 2) * 64 + lane;
-            pn = pn < npts ? pn : npts - 1;
+            if ((s & 63) == 0) {
+                // Fetch chunk k+2 into registers; it is written to LDS 64 steps later.
+                long long pn = (kc + 2) * 64 + lane;
+                pn = pn < npts ? pn : npts - 1;
 #pragma unroll
-            for (int k = 0; k < D; ++k) pre[k] = Q[pn * D + k];
-            if (NS > 0) {
-                const long long ps = pn < cols ? pn : cols - 1;
-                pres[0] = qseg[2 * ps];
-                pres[NS - 1] = qseg[2 * ps + 1];
+                for (int k = 0; k < D; ++k) pre[k] = Q[pn * D + k];
+                if (NS > 0) {
+                    const long long ps = pn < cols ? pn : cols - 1;
+                    pres[0] = qseg[2 * ps];
+                    pres[NS - 1] = qseg[2 * ps + 1];
+                }
             }
             if (skip) {
                 if (has_succ) {
-                    // Columns s-64 (already in lane 0) .. s-1 form a full chunk.
-                    if (lane >= 1) bout = dead_out;
-                    column_store<SYNC>(sout + (s - 64) + lane, bout);
+                    // Columns s-64 (lane s mod 64 of `bout`, output at step
+                    // s-1) .. s-49 form a full 16-column chunk.
+                    if (in_slice && lane != (int)(s & 63)) bout = dead_out;
+                    if (in_slice) column_store<SYNC>(sout + (s & ~63LL) - 64 + lane, bout);
                     __syncthreads();
-                    if (lane == 0) flag_store<SYNC>(&a.progress[band], (int)s);
-                    bout = dead_out;  // column s
+                    if (lane == 0) flag_store<SYNC>(&a.progress[band], (int)(s - 48));
+                    bout = dead_out;  // column s-48
                 }
                 reload = true;
-                s += 63;
+                s += 15;
                 continue;
             }
         }
@@ -538,9 +547,10 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
             const long long col = s - 63;
             const T v = lane_read(out, 63);
             if (lane == (int)(col & 63)) bout = v;
-            if ((col & 63) == 63 || col == cols - 1) {
-                const long long base = col & ~63LL;
-                if (base + lane <= col) column_store<SYNC>(sout + base + lane, bout);
+            if ((col & 15) == 15 || col == cols - 1) {
+                // Columns 16 floor(col / 16) .. col, from their lanes.
+                if ((lane & 48) == (int)(col & 48) && lane <= (int)(col & 63))
+                    column_store<SYNC>(sout + (col & ~63LL) + lane, bout);
                 __syncthreads();
                 if (lane == 0) flag_store<SYNC>(&a.progress[band], (int)(col + 1));
             }
