@@ -8,8 +8,8 @@
 // inputs are its own previous cell (register) and lane l-1's previous output.
 //
 // Bands are taken from an in-order queue (one atomic per band).  Band b reads
-// row 64b-1's outputs from a column buffer that band b-1 fills in chunks of 16
-// columns; band b-1 publishes the count of finished columns with an agent-scope
+// row 64b-1's outputs from a column buffer that band b-1 fills in chunks of
+// kHand columns; band b-1 publishes the count of finished columns with an agent-scope
 // release store, band b polls it with acquire loads.  Because the queue hands
 // out bands in order, every band a wave waits on is already held by a running
 // wave, so the pipeline cannot deadlock whatever the grid size.  Two column
@@ -30,6 +30,12 @@
 namespace rfd {
 
 enum SweepOp { OP_DFD = 0, OP_VEFD = 1, OP_DEC = 2 };
+
+// Columns per band hand-off (a power of two, at most 64): a band trails its
+// predecessor by about 64 + kHand steps, and publishes (a memory wait and a
+// flag store) every kHand columns.  Measured at 200k x 200k and on a batch of
+// 4096 pairs: 16 beats 64 (shorter pipelines) and 8 (the publishes dominate).
+constexpr int kHand = 16;
 
 template <typename T> struct Num;
 template <> struct Num<double> {
@@ -374,7 +380,7 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
 
     for (long long s = 0; s < nsteps; ++s) {
         const long long j = s - lane;
-        if ((s & 15) == 0) {
+        if ((s & (kHand - 1)) == 0) {
             const long long kc = s >> 6;
             if ((s & 63) == 0) {
                 // Stage chunk k+1 (in registers) into LDS.
@@ -386,16 +392,17 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
                     sring[slot][NS - 1][lane] = pres[NS - 1];
                 }
             }
-            // Boundary values of columns s .. s+15, into lanes (s mod 64) ..
-            // (s mod 64) + 15 of `bin` (lane l holds column 64 floor(s / 64) + l),
-            // 16 columns at a time so that a band trails its predecessor by
-            // about 64 + 16 steps, not 64 + 64.  Every lane loads (the other
+            // Boundary values of columns s .. s+kHand-1, into lanes (s mod 64)
+            // .. (s mod 64) + kHand - 1 of `bin` (lane l holds column
+            // 64 floor(s / 64) + l), kHand columns at a time so that a band
+            // trails its predecessor by about 64 + kHand steps, not 64 + 64.
+            // Every lane loads (the other
             // lanes' values are consumed or not yet needed, and are reloaded
             // before use), clamped, so the load count is fixed and the wait for
             // `bin` does not cover the prefetch issued after it.
-            const bool in_slice = (lane & 48) == (int)(s & 48);
+            const bool in_slice = (lane & (64 - kHand)) == (int)(s & (64 - kHand));
             if (has_pred && s < cols) {
-                const long long need = (s + 16 < cols) ? s + 16 : cols;
+                const long long need = (s + kHand < cols) ? s + kHand : cols;
                 while (flag_load<SYNC>(&a.progress[band - 1]) < (int)need)
                     __builtin_amdgcn_s_sleep(1);
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -404,9 +411,9 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
                 if (tlog && lane == 0 && s == 0) tlog[4 * band + 1] = wall_clock64();
             }
             // Dead block: every lane's state and in-flight value is dead and so
-            // is the boundary input for columns s..s+15.  Then every cell of
-            // steps s..s+15 is dead; skip them, publishing dead outputs for
-            // columns s-63..s-48 (lane 63's outputs of those steps).
+            // is the boundary input for columns s..s+kHand-1.  Then every cell
+            // of steps s..s+kHand-1 is dead; skip them, publishing dead outputs
+            // for columns s-63..s-64+kHand (lane 63's outputs of those steps).
             bool skip = false;
             if (OP != OP_DFD && s >= 64 && s + 128 < cols) {
                 bool mine = is_dead<OP>(out, thr);
@@ -431,15 +438,15 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
             if (skip) {
                 if (has_succ) {
                     // Columns s-64 (lane s mod 64 of `bout`, output at step
-                    // s-1) .. s-49 form a full 16-column chunk.
+                    // s-1) .. s-65+kHand form a full chunk.
                     if (in_slice && lane != (int)(s & 63)) bout = dead_out;
                     if (in_slice) column_store<SYNC>(sout + (s & ~63LL) - 64 + lane, bout);
                     __syncthreads();
-                    if (lane == 0) flag_store<SYNC>(&a.progress[band], (int)(s - 48));
-                    bout = dead_out;  // column s-48
+                    if (lane == 0) flag_store<SYNC>(&a.progress[band], (int)(s - 64 + kHand));
+                    bout = dead_out;  // column s-64+kHand
                 }
                 reload = true;
-                s += 15;
+                s += kHand - 1;
                 continue;
             }
         }
@@ -547,9 +554,9 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
             const long long col = s - 63;
             const T v = lane_read(out, 63);
             if (lane == (int)(col & 63)) bout = v;
-            if ((col & 15) == 15 || col == cols - 1) {
-                // Columns 16 floor(col / 16) .. col, from their lanes.
-                if ((lane & 48) == (int)(col & 48) && lane <= (int)(col & 63))
+            if ((col & (kHand - 1)) == kHand - 1 || col == cols - 1) {
+                // Columns kHand floor(col / kHand) .. col, from their lanes.
+                if ((lane & (64 - kHand)) == (int)(col & (64 - kHand)) && lane <= (int)(col & 63))
                     column_store<SYNC>(sout + (col & ~63LL) + lane, bout);
                 __syncthreads();
                 if (lane == 0) flag_store<SYNC>(&a.progress[band], (int)(col + 1));
