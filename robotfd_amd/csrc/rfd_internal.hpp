@@ -19,7 +19,7 @@ struct rfd_ctx {
     int max_waves = 0;
     double wall_khz = 100000.0;   // device wall clock (wall_clock64) rate
     int sync = rfd::SYNC_SCOPED;  // band hand-off protocol (RFD_SYNC=0 selects acquire/release)
-    int rows_per_lane = 4;        // R of band_sweep (RFD_ROWS overrides: 1, 2 or 4)
+    int rows_per_lane = 0;        // R of band_sweep, 1-4 (RFD_ROWS); 0: chosen per sweep
     bool prune = false;           // VE-FD skips cells above DFD^2 (RFD_PRUNE=1 enables)
     bool fused = true;            // FP64 bounds by the fused DFD + VE-FD sweep (RFD_FUSED=0 disables)
     int bounds_cfg = 0;           // fused sweep: 10 * rows per lane + producer waves, 0 = by size (RFD_BOUNDS)
