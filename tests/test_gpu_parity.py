@@ -147,7 +147,7 @@ def pruning_ctx():
 
 @pytest.mark.parametrize("kind", ["walk2", "walk3", "robot"])
 def test_pruned_and_skipped_sweeps_bit_exact(pruning_ctx, kind):
-    """Sizes where VE-FD pruning (cells above DFD^2) and 64-step dead-chunk
+    """Sizes where VE-FD pruning (cells above DFD^2) and 16-step dead-block
     skipping both occur; results must not change."""
     gpu = pruning_ctx
     gpu.reset_stats()
@@ -170,3 +170,34 @@ def test_pruned_and_skipped_sweeps_bit_exact(pruning_ctx, kind):
         assert st["vefd_f64"]["evaluated"] < st["vefd_f64"]["cells"]
         for r in (lb, 0.5 * (lb + gv), gv * (1 - 1e-9), gv, 0.5 * (gv + dfd), dfd):
             assert s.decide_ub(r) == O.decide_ub(P, Q, r)
+
+
+@pytest.mark.parametrize("rows", [1, 2, 4])
+def test_decision_dead_skips_across_bands(rows):
+    """Decisions below and above the distance on pairs of several bands, at 1,
+    2 and 4 rows per lane, where most cells are unreachable: whole 16-step
+    blocks can be dead, skipped, and handed to the next band as dead columns.
+    The terminal rightUB stays bit-exact; dead cells do occur (fewer cells
+    evaluated than swept)."""
+    os.environ["RFD_ROWS"] = str(rows)
+    try:
+        ctx = Context()
+    finally:
+        del os.environ["RFD_ROWS"]
+    try:
+        rng = np.random.default_rng(40 + rows)
+        for d in (2, 3):
+            P, Q = walk(rng, 1100, d), walk(rng, 2300, d)
+            lb = O.endpoint_lb(P, Q)
+            dfd = O.dfd(P, Q)
+            say("dead skips", rows, d)
+            with Intersection(P, Q, ctx=ctx) as s:
+                s.data_move()
+                ctx.reset_stats()
+                for f in (0.2, 0.5, 0.8, 0.95, 1.0, 1.05):
+                    r = lb + f * (dfd - lb)
+                    assert s.decide_ub(r) == O.decide_ub(P, Q, r), (d, f)
+                st = ctx.stats()["decide_f64"]
+                assert st["evaluated"] < st["cells"]
+    finally:
+        ctx.close()
