@@ -34,8 +34,10 @@ enum SweepOp { OP_DFD = 0, OP_VEFD = 1, OP_DEC = 2 };
 // Columns per band hand-off (a power of two, at most 64): a band trails its
 // predecessor by about 64 + kHand steps, and publishes (a memory wait and a
 // flag store) every kHand columns.  Measured at 200k x 200k and on a batch of
-// 4096 pairs: 16 beats 64 (shorter pipelines) and 8 (the publishes dominate).
-constexpr int kHand = 16;
+// 4096 pairs: for decisions and VE-FD 16 beats 64 (shorter pipelines) and 8
+// (the publishes dominate).  DFD, whose cells are cheap and never skipped,
+// keeps 64 (133 ms at 200k x 200k against 145 ms at 16).
+template <int OP> constexpr int kHand = (OP == OP_DFD) ? 64 : 16;
 
 template <typename T> struct Num;
 template <> struct Num<double> {
@@ -380,7 +382,7 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
 
     for (long long s = 0; s < nsteps; ++s) {
         const long long j = s - lane;
-        if ((s & (kHand - 1)) == 0) {
+        if ((s & (kHand<OP> - 1)) == 0) {
             const long long kc = s >> 6;
             if ((s & 63) == 0) {
                 // Stage chunk k+1 (in registers) into LDS.
@@ -400,9 +402,9 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
             // lanes' values are consumed or not yet needed, and are reloaded
             // before use), clamped, so the load count is fixed and the wait for
             // `bin` does not cover the prefetch issued after it.
-            const bool in_slice = (lane & (64 - kHand)) == (int)(s & (64 - kHand));
+            const bool in_slice = (lane & (64 - kHand<OP>)) == (int)(s & (64 - kHand<OP>));
             if (has_pred && s < cols) {
-                const long long need = (s + kHand < cols) ? s + kHand : cols;
+                const long long need = (s + kHand<OP> < cols) ? s + kHand<OP> : cols;
                 while (flag_load<SYNC>(&a.progress[band - 1]) < (int)need)
                     __builtin_amdgcn_s_sleep(1);
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -442,11 +444,11 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
                     if (in_slice && lane != (int)(s & 63)) bout = dead_out;
                     if (in_slice) column_store<SYNC>(sout + (s & ~63LL) - 64 + lane, bout);
                     __syncthreads();
-                    if (lane == 0) flag_store<SYNC>(&a.progress[band], (int)(s - 64 + kHand));
+                    if (lane == 0) flag_store<SYNC>(&a.progress[band], (int)(s - 64 + kHand<OP>));
                     bout = dead_out;  // column s-64+kHand
                 }
                 reload = true;
-                s += kHand - 1;
+                s += kHand<OP> - 1;
                 continue;
             }
         }
@@ -554,9 +556,9 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
             const long long col = s - 63;
             const T v = lane_read(out, 63);
             if (lane == (int)(col & 63)) bout = v;
-            if ((col & (kHand - 1)) == kHand - 1 || col == cols - 1) {
+            if ((col & (kHand<OP> - 1)) == kHand<OP> - 1 || col == cols - 1) {
                 // Columns kHand floor(col / kHand) .. col, from their lanes.
-                if ((lane & (64 - kHand)) == (int)(col & (64 - kHand)) && lane <= (int)(col & 63))
+                if ((lane & (64 - kHand<OP>)) == (int)(col & (64 - kHand<OP>)) && lane <= (int)(col & 63))
                     column_store<SYNC>(sout + (col & ~63LL) + lane, bout);
                 __syncthreads();
                 if (lane == 0) flag_store<SYNC>(&a.progress[band], (int)(col + 1));
