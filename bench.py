@@ -195,9 +195,8 @@ def precision_form(sess, ctx, ref: dict) -> dict:
 
 def robot_section(ctx) -> dict:
     """The long noisy-copy pair (3D robot-like paths of 100,000 and 120,000
-    points, seed 0): its bracket stays open, so the run is the fused bound
-    sweep plus about thirty FP64 decision sweeps of FP64 bisection to 1e-9
-    relative."""
+    points, seed 0), in FP64 to 1e-9 relative.  Its bracket collapses (VE-FD
+    equals DFD), so the run is the fused bound sweep and no decision."""
     from robotfd_amd import Intersection, workloads as W
     P, Q = W.robot_pair(100_000, 120_000, seed=0)
     with Intersection(P, Q, ctx=ctx) as s:

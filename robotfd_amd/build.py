@@ -8,6 +8,7 @@ with packed FP32 ops disabled.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -45,10 +46,18 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+def _jobs() -> int:
+    for var in ("MAX_JOBS", "RFD_BUILD_JOBS"):
+        if os.environ.get(var, "").isdigit():
+            return max(1, int(os.environ[var]))
+    return max(1, min(8, os.cpu_count() or 1))
+
+
 def build(verbose: bool = False) -> str:
-    objs = []
     obj_dir = os.path.join(PKG, "_build")
     os.makedirs(obj_dir, exist_ok=True)
+    cmds = []
+    objs = []
     for src in HIP_SOURCES + CXX_SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
@@ -58,10 +67,15 @@ def build(verbose: bool = False) -> str:
         if src.endswith(".cpp"):
             cmd = [hipcc(), "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
                    f"-I{INCLUDE}", f"-I{CSRC}", "-c", path, "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append(cmd)
         objs.append(obj)
+    # The translation units are independent: compile them side by side.
+    with concurrent.futures.ThreadPoolExecutor(max_workers=_jobs()) as pool:
+        for cmd in cmds:
+            if verbose:
+                print(" ".join(cmd), flush=True)
+        for fut in [pool.submit(subprocess.run, cmd, check=True) for cmd in cmds]:
+            fut.result()
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:
@@ -75,11 +89,6 @@ def build(verbose: bool = False) -> str:
                f"-L{PKG}", "-lrfd", f"-Wl,-rpath,$ORIGIN"]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
-    # Copy kernel of known size that calibrates rocprofv3's byte counters.
-    probe = os.path.join(CSRC, "copy_probe.hip")
-    if os.path.exists(probe):
-        cmd = [hipcc(), "--offload-arch=gfx950", "-O3", probe, "-o", os.path.join(PKG, "copy_probe")]
         subprocess.run(cmd, check=True)
     return LIB
 

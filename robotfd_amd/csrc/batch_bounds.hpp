@@ -43,6 +43,7 @@ __global__ __launch_bounds__(64 * (NPW + 2)) void batch_bounds(BatchArgs a) {
         b.nbands = (int)((p.m + 64LL * R - 1) / (64LL * R));
         b.r2 = p.endpoint_lb * p.endpoint_lb;
         b.tlog = nullptr;
+        b.watch = a.watch;
         if (p.dims <= 2) {
             bounds_band<2, R, NPW, 1, 3>(b, it.y, sh);
         } else {

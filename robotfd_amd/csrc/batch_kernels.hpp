@@ -19,7 +19,10 @@
 namespace rfd {
 __global__ void batch_reset(BatchArgs a) {
     const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (k == 0) *a.t0 = wall_clock64();
+    if (k == 0) {
+        *a.t0 = wall_clock64();
+        *a.watch.abort = 0;
+    }
     if (k >= a.npairs) return;
     BatchState &st = a.state[k];
     const BatchPair &p = a.pairs[k];
@@ -204,6 +207,7 @@ __global__ __launch_bounds__(64) void batch_sweep(BatchArgs a) {
         job.nbands = (int)((rows + 64LL * R - 1) / (64LL * R));
         job.param = (OP == OP_DEC) ? st.radius[node] : (OP == OP_VEFD ? p.endpoint_lb : 0.0);
         job.thr = (OP == OP_VEFD) ? st.thr : INFINITY;
+        job.watch = a.watch;
         // A decision item runs the interval form of its pair (wave-uniform).
         if (OP == OP_DEC && p.dec_form == DEC_TWO_PHASE)
             run_band<OP, 3, double, SYNC_SCOPED, R, true>(job, band, qring, sring, evals, nullptr);

@@ -78,6 +78,7 @@ struct BatchArgs {
     int gate;                // 1: VE-FD only when m*n > 10000
     int max_waves;           // waves of a sweep launch (sets the speculation depth)
     int max_depth;           // 1 = plain sequential bisection
+    Watch watch;             // bound on the hand-off waits (abort word cleared by batch_reset)
 };
 
 __device__ inline bool batch_keep(double lb, double ub, int stop_mode, double eps) {

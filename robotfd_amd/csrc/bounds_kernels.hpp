@@ -60,6 +60,7 @@ struct BoundsArgs {
     int nbands;
     double r2;             // endpoint bound squared: VE-FD's entry value
     unsigned long long *tlog;  // optional, 4 per band (as band_sweep)
+    Watch watch;           // bound on the hand-off waits
 };
 
 // Workgroup hand-off words in LDS.  Relaxed atomics (not volatile) so that,
@@ -265,15 +266,6 @@ __device__ inline void chain_rows(const double (&wv)[C][ChainPlan<R, C, G>::GMAX
         }
 }
 
-// LDS words of the fused decision sweep's hand-offs (dec_kernels.hpp).
-struct BoundsSync {
-    int *prod;       // [NPW] steps each producer has written
-    int *cons;       // steps the chain has consumed
-    int *in_avail;   // incoming columns in the ring
-    int *out_avail;  // outgoing columns the chain has written to the ring
-    int *out_done;   // outgoing columns the courier has published
-};
-
 // Calls f(integral_constant<int, i>) for i = I .. N - 1.
 template <int I, int N, typename F>
 __device__ inline void static_for(F &&f) {
@@ -383,11 +375,19 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
         const int np = (s + 2 < nsteps) ? s + 2 : nsteps;
         const int ni = (C * (s + 2) < cols) ? C * (s + 2) : cols;
         const int no = C * (s - 62) - KB;
+        unsigned polls = 0;
         for (;;) {
             avail = read_prod();
             in_known = __builtin_amdgcn_readfirstlane(lds_get(&sh.in_avail));
             out_known = __builtin_amdgcn_readfirstlane(lds_get(&sh.out_done));
             if (avail >= np && in_known >= ni && (!has_succ || out_known >= no)) break;
+            if (watch_expired(a.watch, polls)) {
+                // Given up (Watch): counts that never hold the chain again.
+                avail = nsteps;
+                in_known = cols;
+                out_known = NONE;
+                break;
+            }
             // A longer nap: a chain that polls fast takes issue slots and LDS
             // reads from the producers it waits for.
             __builtin_amdgcn_s_sleep(8);
@@ -573,6 +573,7 @@ __device__ inline void bounds_courier(const BoundsArgs &a, const int band, Bound
     const bool first = band == 0;
     int in_next = 0;
     int out_next = band + 1 < a.nbands ? 0 : cols;
+    unsigned polls = 0;
     while (in_next < cols || out_next < cols) {
         bool moved = false;
         // Steps the chain has finished: their incoming columns are consumed
@@ -630,7 +631,14 @@ __device__ inline void bounds_courier(const BoundsArgs &a, const int band, Bound
                 moved = true;
             }
         }
-        if (!moved) __builtin_amdgcn_s_sleep(2);
+        if (moved) {
+            polls = 0;
+        } else {
+            // Given up (Watch): the chain and producers run their steps
+            // without this band's columns; the courier's part ends here.
+            if (watch_expired(a.watch, polls)) break;
+            __builtin_amdgcn_s_sleep(2);
+        }
     }
 }
 
@@ -839,9 +847,14 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
             for (int k = 0; k < D; ++k) qn[k] = Q[(long long)cn * DS + k];
         }
         if (s + ahead >= cons_seen + KS) {
+            unsigned polls = 0;
             for (;;) {
                 cons_seen = __builtin_amdgcn_readfirstlane(lds_get(cons));
                 if (s + ahead < cons_seen + KS) break;
+                if (watch_expired(a.watch, polls)) {
+                    cons_seen = 0x7fff0000;  // given up (Watch): never wait for the chain again
+                    break;
+                }
                 __builtin_amdgcn_s_sleep(1);
             }
             __atomic_signal_fence(__ATOMIC_SEQ_CST);

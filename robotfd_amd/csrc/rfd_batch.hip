@@ -65,6 +65,7 @@ struct rfd_batch {
     int2 *bitems = nullptr;    // the fused bound round's fixed work list
     int *count = nullptr;
     unsigned long long *evals = nullptr, *t0 = nullptr;
+    int *abort = nullptr;      // raised by a hand-off wait that gave up (rfd::Watch)
     float *feat = nullptr;     // [npairs x 386] FP32 features of the last bounds
     uint16_t *xin = nullptr;   // [npairs x 400] BF16 network input
     float *pred = nullptr;     // [npairs] network output
@@ -121,6 +122,7 @@ rfd::BatchArgs make_args(rfd_batch *b, int rpl, const rfd_opts &o, int prune) {
     a.gate = o.vefd_gate;
     a.max_waves = b->ctx->max_waves;
     a.max_depth = o.spec_depth >= 1 ? std::min(o.spec_depth, rfd::kMaxDepth) : rfd::kMaxDepth;
+    a.watch = rfd::Watch{b->abort, b->ctx->spin_limit};
     return a;
 }
 
@@ -216,9 +218,15 @@ rfd_status run_once(rfd_batch *b, const rfd_opts &o, int rpl, int prune,
     }
     hst.resize((size_t)b->npairs);
     for (int guard = 0;; ++guard) {
+        int habort = 0;
         RFD_HIP(ctx, hipMemcpyAsync(hst.data(), b->state, sizeof(rfd::BatchState) * b->npairs,
                                     hipMemcpyDeviceToHost, ctx->stream));
+        RFD_HIP(ctx, hipMemcpyAsync(&habort, b->abort, sizeof(habort), hipMemcpyDeviceToHost, ctx->stream));
         RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (habort) {
+            set_err(ctx, "batch: a hand-off wait exceeded %u polls and gave up", ctx->spin_limit);
+            return RFD_ERR_STATE;
+        }
         bool any = false;
         for (const auto &s : hst) any = any || s.active;
         if (!any) break;
@@ -268,6 +276,7 @@ rfd_status rfd_batch_destroy(rfd_batch *b) {
     (void)hipFree(b->count);
     (void)hipFree(b->evals);
     (void)hipFree(b->t0);
+    (void)hipFree(b->abort);
     (void)hipFree(b->feat);
     (void)hipFree(b->xin);
     (void)hipFree(b->pred);
@@ -290,6 +299,8 @@ rfd_status rfd_batch_create(rfd_ctx *ctx, int64_t npairs, const double *const *P
             set_err(ctx, "pair %lld: need non-empty curves of dims 2 or 3", (long long)k);
             return RFD_ERR_ARG;
         }
+        if (rfd_status fs = rfd_host::check_finite(ctx, P[k], m[k], Q[k], n[k], dims[k], k); fs != RFD_OK)
+            return fs;
     }
     RFD_HIP(ctx, hipSetDevice(ctx->device));
     rfd_batch *b = new rfd_batch();
@@ -387,6 +398,8 @@ rfd_status rfd_batch_create(rfd_ctx *ctx, int64_t npairs, const double *const *P
     if ((e = hipMalloc(&b->count, sizeof(int) * 2)) != hipSuccess) return fail(e, "hipMalloc(count)");
     if ((e = hipMalloc(&b->evals, sizeof(unsigned long long))) != hipSuccess) return fail(e, "hipMalloc(evals)");
     if ((e = hipMalloc(&b->t0, sizeof(unsigned long long))) != hipSuccess) return fail(e, "hipMalloc(t0)");
+    if ((e = hipMalloc(&b->abort, sizeof(int))) != hipSuccess) return fail(e, "hipMalloc(abort)");
+    if ((e = hipMemset(b->abort, 0, sizeof(int))) != hipSuccess) return fail(e, "hipMemset(abort)");
     if ((e = hipMalloc(&b->feat, sizeof(float) * npairs * rfd::MLP_FEAT_ROW)) != hipSuccess) return fail(e, "hipMalloc(features)");
     if ((e = hipMalloc(&b->xin, sizeof(uint16_t) * npairs * rfd::MLP_IN_ROW)) != hipSuccess) return fail(e, "hipMalloc(net input)");
     if ((e = hipMalloc(&b->pred, sizeof(float) * npairs)) != hipSuccess) return fail(e, "hipMalloc(predictions)");

@@ -9,7 +9,6 @@
 #include "rfd_internal.hpp"
 #include "sweep_kernels.hpp"
 #include "bounds_kernels.hpp"
-#include "dec_kernels.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -36,6 +35,7 @@ struct rfd_session {
     int *queue = nullptr;
     double *result = nullptr;            // [0] terminal value, [1] its square; fused bounds: [0..3]
     unsigned long long *evals = nullptr; // cells evaluated by the last sweep
+    int *abort = nullptr;                // raised by a hand-off wait that gave up (rfd::Watch)
     double endpoint_lb = 0.0;
     double dfd_sq = -1.0;                // DFD^2 of this pair once known (prunes VE-FD)
     int dec_form = 0;                    // rfd::DEC_FUSED or rfd::DEC_TWO_PHASE
@@ -44,7 +44,7 @@ struct rfd_session {
 namespace rfd {
 
 __global__ void sweep_init(int *progress, int nbands, int *queue, double *result, int nres,
-                           double sentinel, unsigned long long *evals) {
+                           double sentinel, unsigned long long *evals, int *abort) {
     const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int stride = (int)(gridDim.x * blockDim.x);
     for (int k = tid; k < nbands; k += stride) progress[k] = 0;
@@ -52,6 +52,7 @@ __global__ void sweep_init(int *progress, int nbands, int *queue, double *result
         *queue = 0;
         for (int k = 0; k < nres; ++k) result[k] = sentinel;
         *evals = 0;
+        *abort = 0;
     }
 }
 
@@ -150,15 +151,16 @@ rfd_status ensure_qseg(rfd_session *s, T **out, const T *Qd) {
 // shorten the pipeline's fill, until bands outnumber what the SIMDs hold.
 // Measured at 20k x 20k, 100k x 120k and 200k x 200k: DFD and VE-FD sweeps are
 // fastest at R = 2 everywhere (200k: 397 ms against 406 at R = 4 and 462 at
-// R = 1); decisions, whose dead cells are skipped, at the fewest rows whose
-// bands number at most two per SIMD (20k: R = 1, 27.3 ms against 41.0 at
-// R = 4; 100k x 120k: R = 1, 121 against 157; 200k: R = 2, 205 against 213).
+// R = 1).  Decisions, whose dead cells are skipped, are fastest at the fewest
+// rows whose bands number at most 1.25 per SIMD, and at no more than 3 rows
+// (200k: R = 3, 181 ms against 198 at R = 2, 225 at R = 4 and 263 at R = 1;
+// 20k: R = 1).
 int auto_rows_per_lane(const rfd_ctx *ctx, int op, int64_t rows) {
     if (op != rfd::OP_DEC) return 2;
     const int64_t simds = 4 * (int64_t)std::max(ctx->cu_count, 1);
-    for (int r = 1; r < 4; ++r)
-        if ((rows + 64 * r - 1) / (64 * r) <= 2 * simds) return r;
-    return 4;
+    for (int r = 1; r < 3; ++r)
+        if ((rows + 64 * r - 1) / (64 * r) * 4 <= 5 * simds) return r;
+    return 3;
 }
 
 // One sweep: init kernel + band kernel, timed with HIP events on the
@@ -189,6 +191,7 @@ rfd_status run_sweep(rfd_session *s, const T *P, const T *Q, const T *qseg, doub
     a.thr = thr;
     a.evals = s->evals;
     a.dec_form = (OP == rfd::OP_DEC) ? s->dec_form : rfd::DEC_FUSED;
+    a.watch = rfd::Watch{s->abort, ctx->spin_limit};
     const double sentinel = -3.0;  // .cu:193 (result = -3 until the terminal cell writes it)
     unsigned long long *tlog = nullptr;
     if (ctx->tlog_path) {
@@ -200,14 +203,16 @@ rfd_status run_sweep(rfd_session *s, const T *P, const T *Q, const T *qseg, doub
     RFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     hipLaunchKernelGGL(rfd::sweep_init, dim3((unsigned)std::min<int64_t>((nb + 255) / 256, 1024)),
                        dim3(256), 0, ctx->stream, s->progress, (int)nb, s->queue, s->result, 2,
-                       sentinel, s->evals);
+                       sentinel, s->evals, s->abort);
     RFD_HIP(ctx, hipGetLastError());
     RFD_HIP(ctx, (launch_sweep<OP, T>(s->dims, grid, ctx->sync, rpl, ctx->stream, a)));
     RFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     double hres[2] = {0, 0};
     unsigned long long hev = 0;
+    int habort = 0;
     RFD_HIP(ctx, hipMemcpyAsync(hres, s->result, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     RFD_HIP(ctx, hipMemcpyAsync(&hev, s->evals, sizeof(hev), hipMemcpyDeviceToHost, ctx->stream));
+    RFD_HIP(ctx, hipMemcpyAsync(&habort, s->abort, sizeof(habort), hipMemcpyDeviceToHost, ctx->stream));
     RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const double h = hres[0];
     if (result_sq) *result_sq = hres[1];
@@ -229,6 +234,11 @@ rfd_status run_sweep(rfd_session *s, const T *P, const T *Q, const T *qseg, doub
             std::fwrite(h_t.data(), sizeof(unsigned long long), h_t.size(), f);
             std::fclose(f);
         }
+    }
+    if (habort) {
+        set_err(ctx, "sweep kernel %d: a hand-off wait exceeded %u polls and gave up", (int)kind,
+                ctx->spin_limit);
+        return RFD_ERR_STATE;
     }
     if (h == sentinel) {
         set_err(ctx, "sweep kernel %d wrote no terminal value", (int)kind);
@@ -274,6 +284,7 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
     a.cols = cols;
     a.nbands = (int)nb;
     a.r2 = init * init;
+    a.watch = rfd::Watch{s->abort, ctx->spin_limit};
     const double sentinel = -3.0;
     unsigned long long *tlog = nullptr;
     if (ctx->tlog_path) {
@@ -284,13 +295,15 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
     RFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     hipLaunchKernelGGL(rfd::sweep_init, dim3((unsigned)std::min<int64_t>((nb + 255) / 256, 1024)),
                        dim3(256), 0, ctx->stream, s->progress, (int)nb, s->queue, s->result, 4,
-                       sentinel, s->evals);
+                       sentinel, s->evals, s->abort);
     RFD_HIP(ctx, hipGetLastError());
     int grid = 0;
     RFD_HIP(ctx, rfd_launch_bounds(ctx, s->dims, cfg, nb, a, &grid));
     RFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     double hres[4] = {0, 0, 0, 0};
+    int habort = 0;
     RFD_HIP(ctx, hipMemcpyAsync(hres, s->result, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RFD_HIP(ctx, hipMemcpyAsync(&habort, s->abort, sizeof(habort), hipMemcpyDeviceToHost, ctx->stream));
     RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0;
     RFD_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -312,6 +325,10 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
             std::fclose(f);
         }
     }
+    if (habort) {
+        set_err(ctx, "fused bound sweep: a hand-off wait exceeded %u polls and gave up", ctx->spin_limit);
+        return RFD_ERR_STATE;
+    }
     if (hres[0] == sentinel || hres[2] == sentinel) {
         set_err(ctx, "fused bound sweep wrote no terminal value");
         return RFD_ERR_STATE;
@@ -319,76 +336,6 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
     *dfd = hres[0];
     *dfd_sq = hres[1];
     *vefd = hres[2];
-    return RFD_OK;
-}
-
-// Fused decision sweep (dec_kernels.hpp): configurations as the bound sweep.
-template <int D, int R, int NPW>
-hipError_t launch_dec_cfg(rfd_ctx *ctx, int64_t nb, const rfd::BoundsArgs &a) {
-    auto kern = rfd::dec_sweep<D, R, NPW>;
-    int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * (NPW + 2), 0);
-    if (e != hipSuccess) return e;
-    int64_t grid = std::min<int64_t>(nb, (int64_t)ctx->cu_count * std::max(per_cu, 1));
-    grid = std::max<int64_t>(1, std::min<int64_t>(grid, ctx->max_waves));
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * (NPW + 2)), 0, ctx->stream, a);
-    return hipGetLastError();
-}
-
-bool valid_dec_cfg(int cfg) { return cfg == 21 || cfg == 42 || cfg == 63 || cfg == 137; }
-
-template <int D>
-hipError_t launch_dec(rfd_ctx *ctx, int cfg, int64_t nb, const rfd::BoundsArgs &a) {
-    switch (cfg) {
-        case 21: return launch_dec_cfg<D, 2, 1>(ctx, nb, a);
-        case 63: return launch_dec_cfg<D, 6, 3>(ctx, nb, a);
-        case 137: return launch_dec_cfg<D, 13, 7>(ctx, nb, a);
-        default: return launch_dec_cfg<D, 4, 2>(ctx, nb, a);
-    }
-}
-
-// One fused decision sweep at `radius`: the terminal rightUB (.cu:1592).
-rfd_status run_dec(rfd_session *s, double radius, double *result) {
-    rfd_ctx *ctx = s->ctx;
-    const int64_t rows = s->m - 1, cols = s->n - 1;
-    const int cfg = ctx->dec_cfg;
-    const int R = cfg / 10;
-    const int64_t nb = (rows + 64 * R - 1) / (64 * R);
-    rfd::BoundsArgs a{};
-    a.P = s->dP;
-    a.Q = s->dQ;
-    a.slots = (double *)s->slots;
-    a.progress = s->progress;
-    a.queue = s->queue;
-    a.result = s->result;
-    a.rows = rows;
-    a.cols = cols;
-    a.nbands = (int)nb;
-    a.r2 = radius * radius;
-    a.tlog = nullptr;
-    const double sentinel = -3.0;
-    RFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    hipLaunchKernelGGL(rfd::sweep_init, dim3((unsigned)std::min<int64_t>((nb + 255) / 256, 1024)),
-                       dim3(256), 0, ctx->stream, s->progress, (int)nb, s->queue, s->result, 1,
-                       sentinel, s->evals);
-    RFD_HIP(ctx, hipGetLastError());
-    RFD_HIP(ctx, s->dims == 2 ? launch_dec<2>(ctx, cfg, nb, a) : launch_dec<3>(ctx, cfg, nb, a));
-    RFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    double h = 0;
-    RFD_HIP(ctx, hipMemcpyAsync(&h, s->result, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    float ms = 0;
-    RFD_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    const double cells = (double)rows * (double)cols;
-    ctx->stats.launches[RFD_K_DEC64] += 1;
-    ctx->stats.ms[RFD_K_DEC64] += ms;
-    ctx->stats.cells[RFD_K_DEC64] += cells;
-    ctx->stats.evaluated[RFD_K_DEC64] += cells;
-    if (h == sentinel) {
-        set_err(ctx, "fused decision sweep wrote no terminal value");
-        return RFD_ERR_STATE;
-    }
-    *result = h;
     return RFD_OK;
 }
 
@@ -494,9 +441,9 @@ rfd_status rfd_create(rfd_ctx **out, int device) {
         if (rfd_valid_bounds_cfg(c)) ctx->bounds_cfg = c;
     }
     if (const char *e = std::getenv("RFD_WGCU")) ctx->wg_per_cu = std::max(0, std::atoi(e));
-    if (const char *e = std::getenv("RFD_DEC")) {
-        const int c = std::atoi(e);
-        if (valid_dec_cfg(c)) ctx->dec_cfg = c;
+    if (const char *e = std::getenv("RFD_SPIN_LIMIT")) {
+        const long long v = std::atoll(e);
+        ctx->spin_limit = (unsigned)std::max(1LL, std::min(v, 0xffffffffLL));
     }
     if (const char *e = std::getenv("RFD_BATCH_BOUNDS")) {
         const int c = std::atoi(e);
@@ -552,6 +499,7 @@ rfd_status rfd_session_destroy(rfd_session *s) {
     (void)hipFree(s->queue);
     (void)hipFree(s->result);
     (void)hipFree(s->evals);
+    (void)hipFree(s->abort);
     delete s;
     return RFD_OK;
 }
@@ -569,6 +517,7 @@ rfd_status rfd_session_create(rfd_ctx *ctx, const double *P, int64_t m, const do
         set_err(ctx, "curves longer than 2^30 points are not supported");
         return RFD_ERR_ARG;
     }
+    if (rfd_status fs = rfd_host::check_finite(ctx, P, m, Q, n, dims, -1); fs != RFD_OK) return fs;
     RFD_HIP(ctx, hipSetDevice(ctx->device));
     rfd_session *s = new rfd_session();
     s->ctx = ctx;
@@ -592,6 +541,7 @@ rfd_status rfd_session_create(rfd_ctx *ctx, const double *P, int64_t m, const do
     if ((e = hipMalloc(&s->queue, sizeof(int))) != hipSuccess) return fail(e, "hipMalloc(queue)");
     if ((e = hipMalloc(&s->result, 4 * sizeof(double))) != hipSuccess) return fail(e, "hipMalloc(result)");
     if ((e = hipMalloc(&s->evals, sizeof(unsigned long long))) != hipSuccess) return fail(e, "hipMalloc(evals)");
+    if ((e = hipMalloc(&s->abort, sizeof(int))) != hipSuccess) return fail(e, "hipMalloc(abort)");
     if ((e = hipMemcpyAsync(s->dP, P, sizeof(double) * m * dims, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
         return fail(e, "hipMemcpy(P)");
     if ((e = hipMemcpyAsync(s->dQ, Q, sizeof(double) * n * dims, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
@@ -659,11 +609,6 @@ rfd_status rfd_session_decide(rfd_session *s, double radius, int fp32, int *reac
         if ((st = ensure_f32(s)) != RFD_OK) return st;
         if ((st = ensure_qseg<float>(s, &s->qseg32, s->fQ)) != RFD_OK) return st;
         st = run_sweep<rfd::OP_DEC, float>(s, s->fP, s->fQ, s->qseg32, radius, RFD_K_DEC32, &ub);
-    } else if (ctx->fused && ctx->dec_cfg != 0 && s->dec_form == rfd::DEC_FUSED) {
-        // Opt-in (RFD_DEC): at 200k x 200k near the critical radius it takes
-        // 410 ms against 217 ms for band_sweep, which skips the free
-        // intervals of unreachable cells (about 80% of them there).
-        st = run_dec(s, radius, &ub);
     } else {
         if ((st = ensure_qseg<double>(s, &s->qseg64, s->dQ)) != RFD_OK) return st;
         st = run_sweep<rfd::OP_DEC, double>(s, s->dP, s->dQ, s->qseg64, radius, RFD_K_DEC64, &ub);

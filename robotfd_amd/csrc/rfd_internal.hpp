@@ -24,7 +24,7 @@ struct rfd_ctx {
     bool fused = true;            // FP64 bounds by the fused DFD + VE-FD sweep (RFD_FUSED=0 disables)
     int bounds_cfg = 0;           // fused sweep: 10 * rows per lane + producer waves, 0 = by size (RFD_BOUNDS)
     int wg_per_cu = 0;            // fused sweep: cap on workgroups per CU, 0 = none (RFD_WGCU)
-    int dec_cfg = 0;              // fused decision sweep configuration, 0 = off (RFD_DEC opts in)
+    unsigned spin_limit = 1u << 24;  // polls one hand-off wait may take before it gives up (RFD_SPIN_LIMIT)
     int batch_bounds = 1310;      // batches: fused bound round configuration, 0 = separate rounds (RFD_BATCH_BOUNDS)
     int lds_per_cu = 0;           // bytes of LDS per CU
     const char *tlog_path = nullptr;  // RFD_TLOG: append per-band timestamps of every sweep
@@ -92,6 +92,30 @@ inline double endpoint_bound(const double *P, int64_t m, const double *Q, int64_
     double a = d2(P, Q);
     double b = d2(P + (m - 1) * dims, Q + (n - 1) * dims);
     return std::sqrt(std::max(a, b));
+}
+
+
+// Rejects curves holding a NaN or infinite coordinate: the sweeps' min / max
+// recursions and the fused bound sweep (compiled without NaN semantics) assume
+// finite points.  pair >= 0 names a batch pair in the message.
+inline rfd_status check_finite(rfd_ctx *ctx, const double *P, int64_t m, const double *Q, int64_t n,
+                               int dims, int64_t pair) {
+    for (int c = 0; c < 2; ++c) {
+        const double *X = c ? Q : P;
+        const int64_t len = (c ? n : m) * dims;
+        for (int64_t k = 0; k < len; ++k) {
+            if (!std::isfinite(X[k])) {
+                if (pair >= 0)
+                    set_err(ctx, "pair %lld: curve %c point %lld holds a non-finite coordinate", (long long)pair,
+                            c ? 'Q' : 'P', (long long)(k / dims));
+                else
+                    set_err(ctx, "curve %c point %lld holds a non-finite coordinate", c ? 'Q' : 'P',
+                            (long long)(k / dims));
+                return RFD_ERR_ARG;
+            }
+        }
+    }
+    return RFD_OK;
 }
 
 }  // namespace rfd_host

@@ -137,67 +137,84 @@ __device__ inline T point_seg2(const T *x, const T *a, const T *b) {
     return w;
 }
 
-// Free interval of segment [s0, s1] around vertex c (.cu:1173-1200).
-template <typename T, int D>
-__device__ inline void free_iv(const T *c, const T *s0, const T *s1, T r2, T A4, T inv2A,
-                               T &lb, T &ub) {
-    using N = Num<T>;
-    T B = T(0), C = -r2;
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-        T di = s1[k] - s0[k];
-        T mi = s0[k] - c[k];
-        B = N::fma(mi, di, B);
-        C = N::fma(mi, mi, C);
-    }
-    B = B * T(2);
-    T disc = N::fma(B, B, -(A4 * C));
-    lb = T(-1);
-    ub = T(-2);
-    if (disc >= T(0)) {
-        T s = N::sqrt(disc);
-        T ta = (-B - s) * inv2A;
-        T tb = (-B + s) * inv2A;
-        bool invalid = tb < T(0) || ta > T(1);
-        lb = invalid ? T(-1) : N::max(ta, T(0));
-        ub = invalid ? T(-2) : N::min(tb, T(1));
-    }
-}
-
-// The same interval as the reference's two-phase decision path forms it
-// (interval_calculation<D>, .cu:722-782): the C term is (sum of mi^2) - r2,
-// the sum started at 0 (.cu:733-742), and the clamps are comparisons
-// (.cu:751-752), so a NaN end (0 * inf at a zero-length segment) stays NaN.
-template <typename T, int D>
-__device__ inline void free_iv_2ph(const T *c, const T *s0, const T *s1, T r2, T A4, T inv2A,
-                                   T &lb, T &ub) {
-    using N = Num<T>;
-    T B = T(0), mm = T(0);
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-        T di = s1[k] - s0[k];
-        T mi = s0[k] - c[k];
-        B = N::fma(mi, di, B);
-        mm = N::fma(mi, mi, mm);
-    }
-    B = B * T(2);
-    const T C = mm - r2;
-    T disc = N::fma(B, B, -(A4 * C));
-    lb = T(-1);
-    ub = T(-2);
-    if (disc >= T(0)) {
-        T s = N::sqrt(disc);
-        T ta = (-B - s) * inv2A;
-        T tb = (-B + s) * inv2A;
-        if (!(tb < T(0) || ta > T(1))) {
-            lb = (ta < T(0)) ? T(0) : ta;
-            ub = (tb > T(1)) ? T(1) : tb;
-        }
-    }
-}
-
 // Decision interval forms (SweepArgs::dec_form).
 enum DecForm { DEC_FUSED = 0, DEC_TWO_PHASE = 1 };
+
+// Free interval [lb, ub] of segment [s0, s1] around vertex c at radius^2 r2,
+// with A4 = 4|s1 - s0|^2 and inv2A = 0.5 / |s1 - s0|^2 (the per-segment
+// terms); (-1, -2) when the segment misses the disc or the interval leaves
+// [0, 1].  Two forms, as the reference's two decision paths compute it:
+//  fused (TWO_PHASE = false), reachable<D> (.cu:1173-1200): the C term is a
+//    sum started at -r2, and the ends are clamped with max / min;
+//  two-phase (true), interval_calculation<D> (.cu:722-782): the C term is
+//    (sum of mi^2) - r2 (.cu:733-742), and the clamps are comparisons
+//    (.cu:751-752), so a NaN end (0 * inf at a zero-length segment) stays NaN.
+// In three phases (iv_disc, iv_root, iv_ends) so that a caller can run the
+// phases of all its intervals side by side, and without branches: the square
+// root is taken of disc where disc >= 0 (sign of a zero included) and of 0
+// elsewhere, and where disc < 0 selects give (-1, -2), the values a branch
+// around the root gives.
+template <typename T, int D, bool TWO_PHASE>
+__device__ inline void iv_disc(const T *c, const T *s0, const T *s1, T r2, T A4, T &B, T &disc) {
+    using N = Num<T>;
+    T b = T(0), C = TWO_PHASE ? T(0) : -r2;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        T di = s1[k] - s0[k];
+        T mi = s0[k] - c[k];
+        b = N::fma(mi, di, b);
+        C = N::fma(mi, mi, C);
+    }
+    B = b * T(2);
+    if (TWO_PHASE) C = C - r2;
+    disc = N::fma(B, B, -(A4 * C));
+}
+
+template <typename T>
+__device__ inline T iv_root(T disc) {
+    return Num<T>::sqrt(disc >= T(0) ? disc : T(0));
+}
+
+template <typename T, bool TWO_PHASE>
+__device__ inline void iv_ends(T B, T disc, T s, T inv2A, T &lb, T &ub) {
+    using N = Num<T>;
+    const T ta = (-B - s) * inv2A;
+    const T tb = (-B + s) * inv2A;
+    // Bitwise tests: with && the compiler sinks the root and the ends into a
+    // branch on disc >= 0, which keeps the cells' chains from interleaving.
+    const bool valid = (disc >= T(0)) & !((tb < T(0)) | (ta > T(1)));
+    if (TWO_PHASE) {
+        lb = valid ? ((ta < T(0)) ? T(0) : ta) : T(-1);
+        ub = valid ? ((tb > T(1)) ? T(1) : tb) : T(-2);
+    } else {
+        lb = valid ? N::max(ta, T(0)) : T(-1);
+        ub = valid ? N::min(tb, T(1)) : T(-2);
+    }
+}
+
+// Bounded hand-off waits.  Every wait of a sweep counts its polls; at `limit`
+// polls it raises the launch's abort word and gives up.  A wait also gives up
+// when it sees the word raised (read every 64th poll, so a wait that succeeds
+// within 63 polls never touches it), and a wave that gave up waits no more in
+// its band: it runs its remaining steps on whatever values are in its buffers
+// (every index stays within its ring or column), so every wave ends and the
+// kernel exits.  The host reads the word after the launch and reports the
+// sweep as failed.
+struct Watch {
+    int *abort;       // 0 until a wait gives up
+    unsigned limit;   // polls one wait may take (at least 1)
+};
+
+// One failed poll of a wait: true when the wait must give up.
+__device__ inline bool watch_expired(const Watch &w, unsigned &polls) {
+    if (!w.abort) return false;  // no word to raise: the launch waits without a bound
+    ++polls;
+    if (polls >= w.limit) {
+        __hip_atomic_store(w.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return true;
+    }
+    return (polls & 63) == 0 && __hip_atomic_load(w.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
 
 struct SweepArgs {
     const void *P;       // AoS points of the row curve (T)
@@ -214,6 +231,7 @@ struct SweepArgs {
     unsigned long long *evals;  // cells whose weights / free intervals were evaluated
     unsigned long long *tlog;   // optional, 4 per band: wave start, first input ready, end, hw id
     int dec_form;        // decision: DEC_FUSED or DEC_TWO_PHASE (selects the kernel variant)
+    Watch watch;         // bound on the hand-off waits
 };
 
 // "Dead" values: those that can no longer change the sweep's result.
@@ -277,6 +295,7 @@ struct SweepJob {
     long long rows, cols;
     int nbands;
     double param, thr;
+    Watch watch;
 };
 
 template <int OP> struct Staged { static constexpr int value = (OP == OP_DEC) ? 2 : 1; };
@@ -296,7 +315,7 @@ template <int OP> struct Staged { static constexpr int value = (OP == OP_DEC) ? 
 // s-l+1: chunks k-1, k, k+1 are live, the fourth slot is free.
 // VAR selects the operation's variant: VE-FD skips cells whose incoming value
 // is at least thr (exact pruning); the decision uses the two-phase interval
-// form (free_iv_2ph).
+// form (iv_disc / iv_ends with TWO_PHASE).
 template <int OP, int D, typename T, int SYNC, int R, bool VAR = false>
 __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4][D][64],
                                 T (&sring)[4][Staged<OP>::value][64], unsigned long long &evals,
@@ -379,6 +398,47 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
 #pragma unroll
     for (int k = 0; k < D; ++k) qcar[k] = qring[0][k][0];
     bool reload = false;
+    bool gave_up = false;  // a wait exceeded its bound: wait no more (see Watch)
+    // Decision: the free intervals of the current step's cells, once computed
+    // (have_iv), R rows each: right edge [rl, ru], top edge [tl, tu].
+    T rl[R], ru[R], tl[R], tu[R];
+    bool have_iv = false;
+    // Decision: the terminal cell's rightUB, held by the lane that owns row
+    // rows - 1 and stored after the band's last step.
+    T term = T(-3);
+    // The 2R free intervals of the lane's cells in column jc (rows i0 + t),
+    // phase by phase so that their square-root chains sit side by side for
+    // the scheduler.  Columns outside the grid read ring slots that hold other
+    // columns' points; those intervals are never used.
+    auto dec_ivs = [&](const long long jc, T (&l1)[R], T (&u1)[R], T (&l2)[R], T (&u2)[R])
+                       __attribute__((always_inline)) {
+      if constexpr (OP == OP_DEC) {
+        const int ra = (int)((jc >> 6) & 3), oa = (int)(jc & 63);
+        const long long jb = jc + 1;
+        const int rb = (int)((jb >> 6) & 3), ob = (int)(jb & 63);
+        T qa[D], qb[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            qa[k] = qring[ra][k][oa];
+            qb[k] = qring[rb][k][ob];
+        }
+        const T A4b = sring[ra][0][oa];
+        const T inv2b = sring[ra][NS - 1][oa];
+        T Bv[2 * R], Dv[2 * R], Sv[2 * R];
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            iv_disc<T, D, VAR>(p[t + 1], qa, qb, r2, A4b, Bv[2 * t], Dv[2 * t]);
+            iv_disc<T, D, VAR>(qb, p[t], p[t + 1], r2, A4a[t], Bv[2 * t + 1], Dv[2 * t + 1]);
+        }
+#pragma unroll
+        for (int i = 0; i < 2 * R; ++i) Sv[i] = iv_root(Dv[i]);
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            iv_ends<T, VAR>(Bv[2 * t], Dv[2 * t], Sv[2 * t], inv2b, l1[t], u1[t]);
+            iv_ends<T, VAR>(Bv[2 * t + 1], Dv[2 * t + 1], Sv[2 * t + 1], inv2a[t], l2[t], u2[t]);
+        }
+      }
+    };
 
     for (long long s = 0; s < nsteps; ++s) {
         const long long j = s - lane;
@@ -405,8 +465,11 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
             const bool in_slice = (lane & (64 - kHand<OP>)) == (int)(s & (64 - kHand<OP>));
             if (has_pred && s < cols) {
                 const long long need = (s + kHand<OP> < cols) ? s + kHand<OP> : cols;
-                while (flag_load<SYNC>(&a.progress[band - 1]) < (int)need)
-                    __builtin_amdgcn_s_sleep(1);
+                unsigned polls = 0;
+                while (!gave_up && flag_load<SYNC>(&a.progress[band - 1]) < (int)need) {
+                    if (watch_expired(a.watch, polls)) gave_up = true;
+                    else __builtin_amdgcn_s_sleep(1);
+                }
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
                 const long long c = ((s & ~63LL) + lane < cols) ? (s & ~63LL) + lane : cols - 1;
                 bin = column_load<SYNC>(sin + c);
@@ -460,7 +523,76 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
         T left = shift_up(out);
         if (lane == 0) left = bval;
 
-        if (j >= 0 && j < cols && i0 < rows) {
+        if (OP == OP_DEC) {
+            // Decision cells (.cu:1173-1244).  A lane can hold a live cell
+            // this step only if its input from below or one of its rows' own
+            // values is not unreachable (< 0); when no lane can, every cell of
+            // the step gives (-1, -2, -1) and the step costs nothing more.
+            // Otherwise the rows' free intervals are propagated up the rows;
+            // the propagation rules give a dead cell's (-1, -2, -1) from any
+            // intervals, so no lane needs a branch of its own.  The intervals
+            // depend on the points only: a live step computes the next step's
+            // (software pipelining), so that only the propagation, a few
+            // selects per row, sits on the step-to-step path.  A step whose
+            // intervals were not computed the step before (the first live step
+            // after dead ones) computes its own first.
+            const bool inr = j >= 0 && j < cols && i0 < rows;
+            bool maybe = inr && !(left < T(0));
+#pragma unroll
+            for (int t = 0; t < R; ++t) maybe = maybe || (inr && !(bot[t] < T(0)));
+            if (__ballot(maybe) != 0) {
+                if (!have_iv || reload) dec_ivs(j, rl, ru, tl, tu);
+                T nrl[R], nru[R], ntl[R], ntu[R];
+                dec_ivs(j + 1, nrl, nru, ntl, ntu);
+                // Every lane propagates (no branch, so the next step's
+                // intervals above and this chain interleave); lanes outside
+                // the grid keep their state.  Rows past the last (padding of
+                // the last band) are propagated too: their values only reach
+                // padding rows above them, and the last band has no successor.
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    const long long it = i0 + t;
+                    const T leftLB = left, botLB = bot[t];
+                    // Cells whose inputs are both unreachable are not
+                    // counted; a NaN input (two-phase form) is not
+                    // unreachable, so the test is on < 0, not on >= 0.
+                    evals += (inr && it < rows && !((leftLB < T(0)) & (botLB < T(0)))) ? 1 : 0;
+                    // (.cu:1231-1244) as selects: where the input along the
+                    // row is unreachable, the interval on the right edge
+                    // starts at the input from below, if that lies in it;
+                    // where the input from below is unreachable, the top
+                    // edge's starts at the input along the row.
+                    const bool bdead = botLB < T(0), ldead = leftLB < T(0);
+                    const bool rinv = (leftLB > ru[t]) | ldead;
+                    const bool tinv = (botLB > tu[t]) | bdead;
+                    const T rmax = N::max(rl[t], leftLB);
+                    const T tmax = N::max(tl[t], botLB);
+                    const T rLB = bdead ? (rinv ? T(-1) : rmax) : rl[t];
+                    const T rUB = (bdead & rinv) ? T(-2) : ru[t];
+                    const T tLB = ldead ? (tinv ? T(-1) : tmax) : tl[t];
+                    bot[t] = inr ? tLB : bot[t];
+                    left = rLB;
+                    term = (inr && it == rows - 1 && j == cols - 1) ? rUB : term;
+                }
+                out = inr ? left : out;
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    rl[t] = nrl[t];
+                    ru[t] = nru[t];
+                    tl[t] = ntl[t];
+                    tu[t] = ntu[t];
+                }
+                have_iv = true;
+            } else {
+                have_iv = false;
+                // Every row of the lane stays at -1 (already so); the terminal
+                // cell, if this is it, has rightUB -2.
+                out = inr ? T(-1) : out;
+                term = (inr && i0 <= rows - 1 && rows - 1 < i0 + R && j == cols - 1) ? T(-2) : term;
+            }
+            reload = false;
+        }
+        if (OP != OP_DEC && j >= 0 && j < cols && i0 < rows) {
             // q_{j+1} from the LDS ring; q_j is last step's q_{j+1}, carried in
             // registers (re-read from the ring after a skipped chunk).
             const int ra = (int)((j >> 6) & 3), oa = (int)(j & 63);
@@ -475,12 +607,6 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
             for (int k = 0; k < D; ++k) {
                 qa[k] = qcar[k];
                 qb[k] = (OP == OP_DFD) ? qa[k] : qring[rb][k][ob];
-            }
-            T A4b = T(0), inv2b = T(0);
-            if (OP == OP_DEC) {
-                const int ss = (int)((j >> 6) & 3), so = (int)(j & 63);
-                A4b = sring[ss][0][so];
-                inv2b = sring[ss][NS - 1][so];
             }
 #pragma unroll
             for (int t = 0; t < R; ++t) {
@@ -513,37 +639,6 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
                         a.result[0] = (double)N::sqrt(N::min(Tv, Rv));
                         a.result[1] = (double)N::min(Tv, Rv);
                     }
-                } else {
-                    const T leftLB = left, botLB = bot[t];
-                    T rLB = T(-1), rUB = T(-2), tLB = T(-1);
-                    // A cell whose inputs are both unreachable yields (-1, -2, -1)
-                    // whatever its intervals; a NaN input (two-phase form) is not
-                    // unreachable, so the test is on < 0, not on >= 0.
-                    if (!(leftLB < T(0) && botLB < T(0))) {
-                        ++evals;
-                        T tUB;
-                        if (VAR) {
-                            free_iv_2ph<T, D>(p[t + 1], qa, qb, r2, A4b, inv2b, rLB, rUB);
-                            free_iv_2ph<T, D>(qb, p[t], p[t + 1], r2, A4a[t], inv2a[t], tLB, tUB);
-                        } else {
-                            free_iv<T, D>(p[t + 1], qa, qb, r2, A4b, inv2b, rLB, rUB);
-                            free_iv<T, D>(qb, p[t], p[t + 1], r2, A4a[t], inv2a[t], tLB, tUB);
-                        }
-                        if (botLB < T(0)) {
-                            bool inv = leftLB > rUB || leftLB < T(0);
-                            T t2 = N::max(rLB, leftLB);
-                            rLB = inv ? T(-1) : t2;
-                            rUB = inv ? T(-2) : rUB;
-                        }
-                        if (leftLB < T(0)) {
-                            bool inv = botLB > tUB || botLB < T(0);
-                            T t2 = N::max(tLB, botLB);
-                            tLB = inv ? T(-1) : t2;
-                        }
-                    }
-                    bot[t] = tLB;
-                    left = rLB;
-                    if (last) *a.result = (double)rUB;
                 }
             }
             out = left;
@@ -566,6 +661,7 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
         }
     }
     if (OP == OP_DFD) evals += (unsigned long long)((i0 < rows) ? ((rows - i0 < R) ? rows - i0 : R) : 0) * cols;
+    if (OP == OP_DEC && i0 <= rows - 1 && rows - 1 < i0 + R) *a.result = (double)term;
     if (tlog && lane == 0) {
         tlog[4 * band + 2] = wall_clock64();
         tlog[4 * band + 3] = ((unsigned long long)__builtin_amdgcn_s_getreg((20 << 0) | (0 << 6) | (31 << 11)) << 32) |
@@ -574,12 +670,15 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
 }
 
 template <int OP, int D, typename T, int SYNC, int R, bool VAR = false>
-__global__ __launch_bounds__(64) void band_sweep(SweepArgs a) {
+// A single-pair sweep launches at most about two waves per SIMD, so the
+// compiler may spend registers on overlapping a step's independent chains
+// rather than on fitting more waves.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void band_sweep(SweepArgs a) {
     __shared__ T qring[4][D][64];
     __shared__ T sring[4][Staged<OP>::value][64];
     const int lane = (int)threadIdx.x;
     SweepJob job{a.P, a.Q, a.qseg, a.slots, a.progress, a.result, a.rows, a.cols, a.nbands,
-                 a.param, a.thr};
+                 a.param, a.thr, a.watch};
     unsigned long long evals = 0;
     for (;;) {
         int band = 0;

@@ -8,8 +8,10 @@ Names and argument meanings follow the reference:
 * ``Intersection`` (``Cuda_intersection``) include/intersection_algorithm_in_parallel.hpp:11-61
 
 Every call goes through the C ABI of librfd.so (include/rfd.h) to the gfx950
-kernels.  ``forceOversize`` is accepted and ignored: the decision sweep is
-always the fused one (the reference's oversize path, .cu:960-1256).
+kernels.  ``forceOversize`` selects the decision path as in the reference:
+without it the reference picks its two-phase path when the interval arrays fit
+in free device memory (.cu:87-96), with it always the fused one
+(.cu:960-1256); both are reproduced bit for bit.
 """
 from __future__ import annotations
 

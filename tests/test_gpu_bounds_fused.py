@@ -163,19 +163,19 @@ DEC_SHAPES = [(2, 2), (2, 3), (3, 2), (5, 7), (65, 66), (130, 200), (300, 257), 
               (1100, 90)]
 
 
-@pytest.mark.parametrize("cfg", [21, 42, 63, 137])
+@pytest.mark.parametrize("rows", [1, 2, 3, 4])
 @pytest.mark.parametrize("dims", [2, 3])
-def test_fused_decision_configs_bit_exact(contexts, cfg, dims):
-    """The fused decision sweep's terminal rightUB against the oracle's, at
-    radii from the endpoint bound to past DFD."""
-    ctx = contexts(RFD_DEC=cfg)
+def test_decision_rows_per_lane_bit_exact(contexts, rows, dims):
+    """The decision sweep's terminal rightUB against the oracle's at every
+    rows-per-lane choice, at radii from the endpoint bound to past DFD."""
+    ctx = contexts(RFD_ROWS=rows)
     ctx.reset_stats()
     calls = 0
     for m, n in DEC_SHAPES:
-        rng = np.random.default_rng(7 * m + n + dims + cfg)
+        rng = np.random.default_rng(7 * m + n + dims + rows)
         P, Q = walk(rng, m, dims), walk(rng, n, dims)
         lb, ub = O.endpoint_lb(P, Q), O.dfd(P, Q)
-        say("fused decision", cfg, dims, (m, n))
+        say("decision", rows, dims, (m, n))
         with Intersection(P, Q, ctx=ctx) as s:
             for r in np.linspace(lb, ub * 1.05 + 1e-12, 7):
                 assert s.decide_ub(r) == O.decide_ub(P, Q, r), (m, n, r)
@@ -183,9 +183,11 @@ def test_fused_decision_configs_bit_exact(contexts, cfg, dims):
     assert ctx.stats()["decide_f64"]["launches"] == calls
 
 
-@pytest.mark.parametrize("cfg", [42, 137])
-def test_fused_decision_near_critical_value_and_wrap(contexts, cfg):
-    ctx = contexts(RFD_DEC=cfg, RFD_MAX_WAVES=3)
+@pytest.mark.parametrize("rows", [1, 3])
+def test_decision_near_critical_value_and_wrap(contexts, rows):
+    """Waves taking several bands each (3 waves), at radii within 1e-9 and
+    1e-6 of the distance."""
+    ctx = contexts(RFD_ROWS=rows, RFD_MAX_WAVES=3)
     P, Q = W.robot_pair(64 * 13 * 4 + 7, 700, seed=2)
     res = O.distance(P, Q, stop_mode=1, eps=1e-12)
     fd = res["distance"]
@@ -194,8 +196,8 @@ def test_fused_decision_near_critical_value_and_wrap(contexts, cfg):
             assert s.decide_ub(fd * f) == O.decide_ub(P, Q, fd * f), f
 
 
-def test_default_decision_sweep_still_bit_exact(contexts):
-    """Without RFD_DEC the decision runs on band_sweep (unreachable cells skipped)."""
+def test_decision_without_fused_bounds_bit_exact(contexts):
+    """With the separate bound sweeps (RFD_FUSED=0) decisions are unchanged."""
     ctx = contexts(RFD_FUSED=0)
     rng = np.random.default_rng(21)
     P, Q = walk(rng, 300, 3), walk(rng, 257, 3)
