@@ -151,6 +151,22 @@ rfd_status rfd_session_distance(rfd_session *s, const rfd_opts *opts, rfd_result
 rfd_status rfd_distance(rfd_ctx *ctx, const double *P, int64_t m, const double *Q, int64_t n,
                         int dims, const rfd_opts *opts, rfd_result *res);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): rfd_distance on curves that are already in device memory
+ * (dP, dQ: device pointers, AoS FP64), allocating nothing.  Every working
+ * buffer lives in a fixed array the library holds on each GPU it runs on until
+ * the process ends (RFD_NOALLOC_BYTES), which limits each curve to
+ * RFD_NOALLOC_MAX_POINTS points (larger curves: RFD_ERR_ARG).  FP64 only
+ * (opts->precision must be RFD_PREC_FP64); RFD_TLOG is ignored.  The result
+ * equals rfd_distance's on the same curves and options.  All calls on one GPU
+ * share the array: calls that overlap on one GPU (other host threads or
+ * contexts) race and must be serialised by the caller. */
+#define RFD_NOALLOC_MAX_POINTS (1 << 19)
+#define RFD_NOALLOC_BYTES (24 * 1024 * 1024 + 65536)
+rfd_status rfd_distance_noalloc(rfd_ctx *ctx, const double *dP, int64_t m, const double *dQ, int64_t n,
+                                int dims, const rfd_opts *opts, rfd_result *res);
+
 /* ---- Batches: many pairs, resident on the device, one launch per round ----
  * No counterpart in the reference, which computes one pair per call.  All
  * rounds (DFD, VE-FD, the bisection's decisions) of all pairs run as

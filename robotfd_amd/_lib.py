@@ -85,6 +85,8 @@ SIGNATURES = {
     "rfd_session_distance": (ctypes.c_int, [_vp, ctypes.POINTER(Opts), ctypes.POINTER(Result)]),
     "rfd_distance": (ctypes.c_int, [_vp, _dp, _i64, _dp, _i64, ctypes.c_int,
                                     ctypes.POINTER(Opts), ctypes.POINTER(Result)]),
+    "rfd_distance_noalloc": (ctypes.c_int, [_vp, _vp, _i64, _vp, _i64, ctypes.c_int,
+                                            ctypes.POINTER(Opts), ctypes.POINTER(Result)]),
     "rfd_batch_create": (ctypes.c_int, [_vp, _i64, ctypes.POINTER(_dp), ctypes.POINTER(_i64),
                                         ctypes.POINTER(_dp), ctypes.POINTER(_i64),
                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(_vp)]),

@@ -14,11 +14,13 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -39,6 +41,7 @@ struct rfd_session {
     double endpoint_lb = 0.0;
     double dfd_sq = -1.0;                // DFD^2 of this pair once known (prunes VE-FD)
     int dec_form = 0;                    // rfd::DEC_FUSED or rfd::DEC_TWO_PHASE
+    bool noalloc = false;                // buffers in the library's fixed array: allocate nothing
 };
 
 namespace rfd {
@@ -61,7 +64,60 @@ __global__ void to_f32(const double *src, float *dst, long long count) {
     if (k < count) dst[k] = (float)src[k];
 }
 
+// flag[0] <- 0 (a kernel, not a memset, so that the call stays capturable).
+__global__ void clear_flag(int *flag) { *flag = 0; }
+
+// flag[0] <- 1 if any of x[0 .. count) is NaN or infinite.
+__global__ void scan_finite(const double *x, long long count, int *flag) {
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < count;
+         k += (long long)gridDim.x * blockDim.x)
+        if (!__builtin_isfinite(x[k])) *flag = 1;
+}
+
 }  // namespace rfd
+
+// ---- Fixed working array of rfd_distance_noalloc ----
+//
+// Every buffer a distance computation needs beyond its inputs and outputs
+// (band boundary columns, the decision's per-column terms, progress flags, the
+// band queue, the terminal values, the evaluated-cell count, the abort word
+// and a flag) lives in this array, so that the entry point allocates nothing.
+// Each GPU the library runs on holds its own copy until the process ends; its
+// address on a device comes from hipGetSymbolAddress with that device current
+// (rfd_create), never from the array's name, which host code sees only as a
+// stand-in.  Layout (bytes): [0, 256) words and terminal values, then the
+// progress flags, the boundary columns (4 n doubles) and the per-column terms
+// (2 (n - 1) doubles) for n <= RFD_NOALLOC_MAX_POINTS.
+namespace {
+constexpr long long kNaMax = RFD_NOALLOC_MAX_POINTS;
+constexpr long long kNaProgressOff = 256;
+constexpr long long kNaProgressBytes = ((kNaMax / 64 + 2) * 4 + 255) / 256 * 256;
+constexpr long long kNaSlotsOff = kNaProgressOff + kNaProgressBytes;
+constexpr long long kNaQsegOff = kNaSlotsOff + 4 * kNaMax * 8;
+constexpr long long kNaBytes = kNaQsegOff + 2 * kNaMax * 8;
+static_assert(kNaBytes <= RFD_NOALLOC_BYTES, "fixed working array too small");
+}  // namespace
+
+static __device__ __align__(256) double scratch[RFD_NOALLOC_BYTES / 8];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[64] = {};  // per device: the array's device address
+
+// The array's address on `device` (current), looked up once per device.
+hipError_t scratch_address(int device, void **out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+}  // namespace
 
 namespace {
 
@@ -194,7 +250,7 @@ rfd_status run_sweep(rfd_session *s, const T *P, const T *Q, const T *qseg, doub
     a.watch = rfd::Watch{s->abort, ctx->spin_limit};
     const double sentinel = -3.0;  // .cu:193 (result = -3 until the terminal cell writes it)
     unsigned long long *tlog = nullptr;
-    if (ctx->tlog_path) {
+    if (ctx->tlog_path && !s->noalloc) {
         RFD_HIP(ctx, hipMalloc(&tlog, sizeof(unsigned long long) * 4 * nb));
         RFD_HIP(ctx, hipMemsetAsync(tlog, 0, sizeof(unsigned long long) * 4 * nb, ctx->stream));
     }
@@ -287,7 +343,7 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
     a.watch = rfd::Watch{s->abort, ctx->spin_limit};
     const double sentinel = -3.0;
     unsigned long long *tlog = nullptr;
-    if (ctx->tlog_path) {
+    if (ctx->tlog_path && !s->noalloc) {
         RFD_HIP(ctx, hipMalloc(&tlog, sizeof(unsigned long long) * 4 * nb));
         RFD_HIP(ctx, hipMemsetAsync(tlog, 0, sizeof(unsigned long long) * 4 * nb, ctx->stream));
     }
@@ -422,6 +478,13 @@ rfd_status rfd_create(rfd_ctx **out, int device) {
         return RFD_ERR_HIP;
     }
     ctx->cu_count = prop.multiProcessorCount;
+    if (scratch_address(device, &ctx->scratch) != hipSuccess) {
+        (void)hipEventDestroy(ctx->ev0);
+        (void)hipEventDestroy(ctx->ev1);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return RFD_ERR_HIP;
+    }
     {
         int khz = 0;
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
@@ -635,10 +698,15 @@ static rfd_status session_distance_impl(rfd_session *s, const rfd_opts *opts, rf
     rfd_ctx *ctx = s->ctx;
     std::memset(res, 0, sizeof(*res));
     RFD_HIP(ctx, hipSetDevice(ctx->device));
-    hipEvent_t t0, t1;
-    RFD_HIP(ctx, hipEventCreate(&t0));
-    RFD_HIP(ctx, hipEventCreate(&t1));
-    RFD_HIP(ctx, hipEventRecord(t0, ctx->stream));
+    // Wall time of the call: HIP events on the stream, or, where nothing may be
+    // created (noalloc), the host clock around the synchronous call.
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    const auto h0 = std::chrono::steady_clock::now();
+    if (!s->noalloc) {
+        RFD_HIP(ctx, hipEventCreate(&t0));
+        RFD_HIP(ctx, hipEventCreate(&t1));
+        RFD_HIP(ctx, hipEventRecord(t0, ctx->stream));
+    }
     if (o.verbose) std::printf("%lld, %lld\n", (long long)s->m, (long long)s->n);  // frechet.cpp:21
 
     double lb = s->endpoint_lb;
@@ -728,12 +796,17 @@ static rfd_status session_distance_impl(rfd_session *s, const rfd_opts *opts, rf
         if (r) ub = split; else lb = split;
     }
     double value = (ub + lb) / 2.0;  // frechet.cpp:63
-    RFD_HIP(ctx, hipEventRecord(t1, ctx->stream));
-    RFD_HIP(ctx, hipEventSynchronize(t1));
     float ms = 0;
-    (void)hipEventElapsedTime(&ms, t0, t1);
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
+    if (!s->noalloc) {
+        RFD_HIP(ctx, hipEventRecord(t1, ctx->stream));
+        RFD_HIP(ctx, hipEventSynchronize(t1));
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+    } else {
+        RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    }
     res->distance = value;
     res->lb = lb;
     res->ub = ub;
@@ -773,6 +846,88 @@ rfd_status rfd_distance(rfd_ctx *ctx, const double *P, int64_t m, const double *
     st = rfd_session_distance(s, opts, res);
     rfd_session_destroy(s);
     return st;
+}
+
+rfd_status rfd_distance_noalloc(rfd_ctx *ctx, const double *dP, int64_t m, const double *dQ, int64_t n,
+                                int dims, const rfd_opts *opts, rfd_result *res) {
+    if (!ctx || !dP || !dQ || !res) return RFD_ERR_ARG;
+    rfd_opts o;
+    if (opts) o = *opts; else rfd_default_opts(&o);
+    if (m < 1 || n < 1 || (dims != 2 && dims != 3)) {
+        set_err(ctx, "need non-empty curves and dims in {2,3} (m=%lld n=%lld dims=%d)", (long long)m,
+                (long long)n, dims);
+        return RFD_ERR_ARG;
+    }
+    if (m > kNaMax || n > kNaMax) {
+        set_err(ctx, "rfd_distance_noalloc: curves of %lld and %lld points exceed its fixed working array "
+                "(at most %lld points per curve)", (long long)m, (long long)n, (long long)kNaMax);
+        return RFD_ERR_ARG;
+    }
+    if (o.precision != RFD_PREC_FP64) {
+        set_err(ctx, "rfd_distance_noalloc computes in FP64 only (precision must be RFD_PREC_FP64)");
+        return RFD_ERR_ARG;
+    }
+    if (o.decide_form < RFD_DECIDE_FUSED || o.decide_form > RFD_DECIDE_REFERENCE) return RFD_ERR_ARG;
+    if (!ctx->scratch) {
+        set_err(ctx, "the fixed working array has no device address");
+        return RFD_ERR_STATE;
+    }
+    RFD_HIP(ctx, hipSetDevice(ctx->device));
+    char *base = static_cast<char *>(ctx->scratch);
+    int *abort = reinterpret_cast<int *>(base);
+    int *queue = reinterpret_cast<int *>(base + 4);
+    int *flag = reinterpret_cast<int *>(base + 8);
+    double *result = reinterpret_cast<double *>(base + 64);
+    unsigned long long *evals = reinterpret_cast<unsigned long long *>(base + 96);
+    // Non-finite coordinates are refused (as by rfd_session_create), found by
+    // a scan on the device; the endpoints come back for the endpoint bound.
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(rfd::clear_flag, dim3(1), dim3(1), 0, st, flag);
+    const long long cp = m * dims, cq = n * dims;
+    hipLaunchKernelGGL(rfd::scan_finite, dim3((unsigned)std::min<long long>((cp + 255) / 256, 1024)), dim3(256), 0,
+                       st, dP, cp, flag);
+    hipLaunchKernelGGL(rfd::scan_finite, dim3((unsigned)std::min<long long>((cq + 255) / 256, 1024)), dim3(256), 0,
+                       st, dQ, cq, flag);
+    RFD_HIP(ctx, hipGetLastError());
+    double ends[12] = {0};  // P first, P last, Q first, Q last
+    int hflag = 0;
+    RFD_HIP(ctx, hipMemcpyAsync(ends, dP, sizeof(double) * dims, hipMemcpyDeviceToHost, st));
+    RFD_HIP(ctx, hipMemcpyAsync(ends + dims, dP + (m - 1) * dims, sizeof(double) * dims, hipMemcpyDeviceToHost, st));
+    RFD_HIP(ctx, hipMemcpyAsync(ends + 2 * dims, dQ, sizeof(double) * dims, hipMemcpyDeviceToHost, st));
+    RFD_HIP(ctx, hipMemcpyAsync(ends + 3 * dims, dQ + (n - 1) * dims, sizeof(double) * dims, hipMemcpyDeviceToHost, st));
+    RFD_HIP(ctx, hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    RFD_HIP(ctx, hipStreamSynchronize(st));
+    if (hflag) {
+        set_err(ctx, "a curve holds a non-finite coordinate");
+        return RFD_ERR_ARG;
+    }
+    // A session over the caller's curves whose buffers are the array's.
+    rfd_session s;
+    s.ctx = ctx;
+    s.m = m;
+    s.n = n;
+    s.dims = dims;
+    s.dP = const_cast<double *>(dP);
+    s.dQ = const_cast<double *>(dQ);
+    s.slots = base + kNaSlotsOff;
+    s.progress = reinterpret_cast<int *>(base + kNaProgressOff);
+    s.queue = queue;
+    s.result = result;
+    s.evals = evals;
+    s.abort = abort;
+    s.endpoint_lb = endpoint_bound(ends, 2, ends + 2 * dims, 2, dims);
+    s.noalloc = true;
+    if (n >= 2) {
+        s.qseg64 = reinterpret_cast<double *>(base + kNaQsegOff);
+        const unsigned g = (unsigned)((n - 1 + 255) / 256);
+        if (dims == 2)
+            hipLaunchKernelGGL((rfd::qseg_terms<2, double>), dim3(g), dim3(256), 0, st, dQ, (long long)(n - 1), s.qseg64);
+        else
+            hipLaunchKernelGGL((rfd::qseg_terms<3, double>), dim3(g), dim3(256), 0, st, dQ, (long long)(n - 1), s.qseg64);
+        RFD_HIP(ctx, hipGetLastError());
+    }
+    s.dec_form = rfd_host::resolve_form(&s, o.decide_form);
+    return session_distance_impl(&s, &o, res);
 }
 
 }  // extern "C"

@@ -28,6 +28,7 @@ struct rfd_ctx {
     int batch_bounds = 1310;      // batches: fused bound round configuration, 0 = separate rounds (RFD_BATCH_BOUNDS)
     int lds_per_cu = 0;           // bytes of LDS per CU
     const char *tlog_path = nullptr;  // RFD_TLOG: append per-band timestamps of every sweep
+    void *scratch = nullptr;      // device address of the library's fixed working array on this device
     rfd_stats stats{};
     char err[512] = {0};
 };
