@@ -213,6 +213,34 @@ def robot_section(ctx) -> dict:
             "kernels": {k: dict(v, avg_ms=v["ms"] / v["launches"]) for k, v in st.items()}}
 
 
+def refined_ub_section(pairs, open_mask, ctx, levels=(1, 2)) -> dict:
+    """What the tighter upper bound (DFD of the midpoint-refined curves) saves
+    on the pairs whose bracket stays open: decision sweeps and time of their
+    single-pair runs from [VE-FD, DFD] and from the refined bracket."""
+    import numpy as np
+    from robotfd_amd import Intersection
+    idx = np.flatnonzero(open_mask)
+    out = {"open_pairs": int(idx.size)}
+    if not idx.size:
+        return out
+    runs = {0: [0, 0.0]}
+    for lv in levels:
+        runs[lv] = [0, 0.0]
+    for k in idx:
+        P, Q = pairs[k]
+        with Intersection(P, Q, ctx=ctx) as s:
+            for lv in runs:
+                r = s.distance(stop="rel", eps=1e-9, refine_ub=lv)
+                runs[lv][0] += int(r["iterations"])
+                runs[lv][1] += r["ms_total"]
+    out["decision_sweeps_plain"] = runs[0][0]
+    out["ms_plain"] = runs[0][1]
+    for lv in levels:
+        out[f"levels_{lv}"] = {"decision_sweeps": runs[lv][0], "sweeps_saved": runs[0][0] - runs[lv][0],
+                               "ms": runs[lv][1]}
+    return out
+
+
 def batch_section(count: int, ctx) -> dict:
     """Pairs/s of the batched pipeline on `count` pairs (lengths log-uniform in
     [500, 20000] per curve, dims 2/3 at random, N(0,1) random walks, seed 0;
@@ -251,6 +279,12 @@ def batch_section(count: int, ctx) -> dict:
             entry = {"pairs_per_s": count / dt, "ms": dt * 1e3,
                      "p50_latency_ms": float(np.percentile(lat, 50)),
                      "p99_latency_ms": float(np.percentile(lat, 99)),
+                     "p99_9_latency_ms": float(np.percentile(lat, 99.9)),
+                     "max_latency_ms": float(lat.max()),
+                     # Pairs whose bracket stays open after the bounds finish in
+                     # the decision rounds: their latencies are the tail.
+                     "open_pair_p50_latency_ms": float(np.percentile(lat[open_pairs], 50)) if open_pairs.any() else None,
+                     "open_pair_max_latency_ms": float(lat[open_pairs].max()) if open_pairs.any() else None,
                      "decision_sweeps": int(r["iterations"].sum()),
                      "max_rel_diff_vs_plain": float(np.max(np.abs(r["distance"] - ref["distance"]) /
                                                            np.maximum(ref["distance"], 1e-300)))}
@@ -263,11 +297,50 @@ def batch_section(count: int, ctx) -> dict:
         for name in ("speculative", "learned", "learned_speculative"):
             if name in res:
                 res[name]["speedup_vs_plain"] = res[name]["pairs_per_s"] / res["plain"]["pairs_per_s"]
+        # The bound round (DFD and VE-FD of every pair) ends where the closed
+        # pairs finish; its share of the plain run caps what any decision-side
+        # speed-up (speculation, the learned bracket) can gain.
+        closed = ref["iterations"] == 0
+        if closed.any():
+            bound_ms = float(ref["latency_ms"][closed].max())
+            res["bound_round_ms"] = bound_ms
+            res["bound_round_frac_of_plain"] = bound_ms / res["plain"]["ms"]
+            res["speedup_ceiling_if_decisions_free"] = res["plain"]["ms"] / bound_ms
+        res["refined_ub"] = refined_ub_section(pairs, ref["iterations"] > 0, ctx)
         if net is not None:
             res["learned_policy"] = {"tau": net.policy[0], "delta": net.policy[1],
                                      "weights": os.path.relpath(DEFAULT_WEIGHTS, ROOT)}
             net.close()
     return res
+
+
+def profile_mode(what: str, points: int) -> int:
+    """One launch of the named kernel (after the bound sweep it needs), for
+    counter runs under rocprofv3; prints what ran."""
+    if what == "copy":
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        nbytes = 256 << 20
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+        assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+        for _ in range(3):
+            assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+        assert hip.hipDeviceSynchronize() == 0
+        print(json.dumps({"profile": "copy", "bytes_read": nbytes, "bytes_written": nbytes, "copies": 3}))
+        return 0
+    from robotfd_amd import Context, Intersection, workloads as W
+    P, Q = W.long_3d(points, points, seed=0)
+    ctx = Context.default()
+    with Intersection(P, Q, ctx=ctx) as s:
+        dfd, ve = s.Bound_call_gpu(0.0)
+        out = {"profile": what, "points": points, "dfd": dfd, "vefd": ve}
+        if what == "decision":
+            out["radius"] = dfd * (1 - 0.5e-9)
+            out["terminal_ub"] = s.decide_ub(out["radius"])
+        out["kernels"] = ctx.stats()
+    print(json.dumps(out))
+    return 0
 
 
 def main() -> int:
@@ -284,7 +357,13 @@ def main() -> int:
                     help="skip the mixed-precision run reported beside the FP64 line")
     ap.add_argument("--batch-pairs", type=int, default=4096,
                     help="pairs of the batched workload reported beside the main line (0: skip)")
+    ap.add_argument("--profile", choices=["decision", "bounds", "copy"],
+                    help="run one launch of a kernel for rocprofv3 and exit: the FP64 decision at "
+                         "DFD (1 - 0.5e-9) or the fused bound sweep on the pair of --points, or a "
+                         "256 MiB device-to-device copy (byte-counter calibration)")
     args = ap.parse_args()
+    if args.profile:
+        return profile_mode(args.profile, args.points)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path has no "
               "multi-GPU sharding in this build", file=sys.stderr)

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstddef>
+#include <cstdio>
 #include <iostream>
 #include <vector>
 
@@ -23,6 +24,15 @@ public:
     Point front() const { return points.front(); }
     Point back() const { return points.back(); }
     void push_back(const Point &p) { points.push_back(p); }
+    // Translates every point by -point, printing the second point before and
+    // after and the offset, as the reference does (curve.hpp:25-30); needs at
+    // least two points.
+    void recenter(const Point &point) {
+        std::printf("%f, %f\n", points[1][0], points[1][1]);
+        std::printf("%f, %f\n", point[0], point[1]);
+        for (auto &p : points) p -= point;
+        std::printf("%f, %f\n", points[1][0], points[1][1]);
+    }
 
     Points::iterator begin() { return points.begin(); }
     Points::iterator end() { return points.end(); }
@@ -38,3 +48,10 @@ class Curves : public std::vector<Curve> {
 public:
     Curve get(std::size_t i) const { return this->operator[](i); }
 };
+
+// The reference declares this printer (curve.hpp:49) without defining it: one
+// point per line, coordinates separated by commas (the loader's format).
+inline std::ostream &operator<<(std::ostream &out, const Curve &curve) {
+    for (const Point &p : curve) out << p << '\n';
+    return out;
+}

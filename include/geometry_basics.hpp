@@ -48,5 +48,13 @@ struct Point {
     int size() const { return (int)coordinates.size(); }
 };
 
+// The reference declares this printer (geometry_basics.hpp:103) without
+// defining it: the coordinates separated by commas.
+#include <ostream>
+inline std::ostream &operator<<(std::ostream &out, const Point &p) {
+    for (std::size_t k = 0; k < p.coordinates.size(); ++k) out << (k ? "," : "") << p.coordinates[k];
+    return out;
+}
+
 typedef std::vector<Point> Points;
 typedef std::vector<Points> multiPoints;

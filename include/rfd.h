@@ -69,6 +69,9 @@ typedef struct {
     int32_t spec_depth;    /* batch: levels of the bisection tree one decision round may
                               evaluate when few pairs remain (0 = library default, 1 = plain) */
     int32_t decide_form;   /* rfd_decide_form of the decisions (default RFD_DECIDE_FUSED) */
+    int32_t refine_ub;     /* single pair: 0 = off (default); k > 0 = when the bracket is open,
+                              lower DFD to the DFD of both curves refined k times at segment
+                              midpoints (rfd_session_refined_ub) before bisecting */
 } rfd_opts;
 
 typedef struct {
@@ -77,7 +80,8 @@ typedef struct {
     double dfd;            /* upper bound from DFD<D> */
     double vefd;           /* lower bound from VEFD<D> (endpoint_lb when gated off) */
     double lb, ub;         /* final bracket */
-    double err_bound;      /* (ub - lb) / 2 / distance: relative error bound of `distance` */
+    double err_bound;      /* max(ub - distance, distance - lb) / lb: bounds |distance - FD| / FD
+                              for the distance FD, which lies in [lb, ub] */
     int64_t iterations;    /* bisection steps (all precisions) */
     int64_t iterations_fp32;
     int64_t certify_steps; /* FP64 decisions spent certifying an FP32 bracket */
@@ -91,6 +95,8 @@ typedef struct {
     double cells[12];      /* grid cells swept (the algorithmic count) */
     double evaluated[12];  /* cells whose weights / free intervals were computed;
                               the rest were skipped as unable to change the result */
+    double skipped_blocks[12];  /* band sweeps: whole blocks of dead steps skipped at once
+                                   (every lane's cells and inputs unreachable or pruned) */
 } rfd_stats;
 
 typedef enum {
@@ -140,6 +146,15 @@ rfd_status rfd_session_decide(rfd_session *s, double radius, int fp32, int *reac
  * *effective (may be NULL) receives the form in use (FUSED or TWO_PHASE).
  * rfd_session_decide uses it; rfd_session_distance uses opts->decide_form. */
 rfd_status rfd_session_set_decide_form(rfd_session *s, int32_t form, int32_t *effective);
+
+/* A tighter upper bound than DFD, an addition beside Bound_call_gpu
+ * (.cu:1262-1326), with no counterpart in the reference: *ub <- DFD of both
+ * curves with every segment split at its midpoint, `levels` times (0..8; the
+ * point a + 0.5 (b - a) between consecutive points a, b).  A coupling of the
+ * refined vertices is a matching of the curves and the distance of two points
+ * moving linearly is convex, so FD <= *ub <= DFD, up to the rounding of the
+ * midpoints.  Costs a DFD sweep of 4^levels times the cells. */
+rfd_status rfd_session_refined_ub(rfd_session *s, int32_t levels, double *ub);
 
 /* Replaces Frechet::Continuous::distance_cuda (frechet.cpp:18-75) on a
  * session whose curves are already resident on the device. */

@@ -74,6 +74,21 @@ inline T point_seg2(const T *x, const T *a, const T *b, int D) {
     return w;
 }
 
+// The interval ends (-B -/+ sqrt(disc)) / (2A), with B, disc and inv2A in T.
+// In FP64 every operation is FP64.  In FP32 (the precision form's early
+// decisions) the square root and the two ends are computed in FP64 from the
+// FP32 B, disc and inv2A, and the ends rounded to FP32 once.
+inline void iv_ends(double B, double disc, double inv2A, double &ta, double &tb) {
+    const double s = std::sqrt(disc);
+    ta = (-B - s) * inv2A;
+    tb = (-B + s) * inv2A;
+}
+inline void iv_ends(float B, float disc, float inv2A, float &ta, float &tb) {
+    const double s = std::sqrt((double)disc);
+    ta = (float)((-(double)B - s) * (double)inv2A);
+    tb = (float)((-(double)B + s) * (double)inv2A);
+}
+
 // Free interval of an edge: the part of segment [s0, s1] within sqrt(r2) of
 // vertex c (.cu:1173-1230).  A4 = 4*|s1-s0|^2 and inv2A = 0.5/|s1-s0|^2 are
 // per-segment terms.  mi = s0 - c, di = s1 - s0 (.cu:1178-1179, 1208-1209).
@@ -92,9 +107,8 @@ inline void free_iv(const T *c, const T *s0, const T *s1, int D, T r2, T A4, T i
     lb = T(-1);
     ub = T(-2);
     if (disc >= T(0)) {
-        T s = sqrtT(disc);
-        T ta = (-B - s) * inv2A;
-        T tb = (-B + s) * inv2A;
+        T ta, tb;
+        iv_ends(B, disc, inv2A, ta, tb);
         bool invalid = tb < T(0) || ta > T(1);
         lb = invalid ? T(-1) : std::fmax(ta, T(0));
         ub = invalid ? T(-2) : std::fmin(tb, T(1));
@@ -124,9 +138,8 @@ inline void free_iv_2ph(const T *c, const T *s0, const T *s1, int D, T r2, T A4,
     lb = T(-1);
     ub = T(-2);
     if (disc >= T(0)) {
-        T s = sqrtT(disc);
-        T ta = (-B - s) * inv2A;
-        T tb = (-B + s) * inv2A;
+        T ta, tb;
+        iv_ends(B, disc, inv2A, ta, tb);
         if (!(tb < T(0) || ta > T(1))) {
             if (ta < T(0)) ta = T(0);
             if (tb > T(1)) tb = T(1);
@@ -309,6 +322,42 @@ double or_endpoint_lb(const double *P, int64_t m, const double *Q, int64_t n, in
     return std::sqrt(std::max(a, b));
 }
 
+// Midpoint refinement, `levels` times: between consecutive points a, b the
+// point a + 0.5 (b - a) (0.5 (b - a) is exact, so one rounding).  out holds
+// (m - 1) 2^levels + 1 points.
+int64_t or_refine(const double *P, int64_t m, int dims, int levels, double *out) {
+    std::vector<double> cur(P, P + m * dims), nxt;
+    int64_t len = m;
+    for (int l = 0; l < levels && len >= 2; ++l) {
+        nxt.assign((size_t)((2 * len - 1) * dims), 0.0);
+        for (int64_t i = 0; i + 1 < len; ++i)
+            for (int k = 0; k < dims; ++k) {
+                const double a = cur[(size_t)(i * dims + k)], b = cur[(size_t)((i + 1) * dims + k)];
+                nxt[(size_t)(2 * i * dims + k)] = a;
+                nxt[(size_t)((2 * i + 1) * dims + k)] = a + 0.5 * (b - a);
+            }
+        for (int k = 0; k < dims; ++k) nxt[(size_t)((2 * len - 2) * dims + k)] = cur[(size_t)((len - 1) * dims + k)];
+        cur.swap(nxt);
+        len = 2 * len - 1;
+    }
+    if (out) std::copy(cur.begin(), cur.end(), out);
+    return len;
+}
+
+// Upper bound of the continuous distance at most DFD: DFD of both curves
+// refined `levels` times.  A coupling of the refined vertex sequences is a
+// matching of the curves, and the distance of two points moving linearly is
+// convex, so each refined pair is at most the larger of the pairs it lies
+// between: FD <= this <= DFD (up to the rounding of the midpoints).
+double or_dfd_refined(const double *P, int64_t m, const double *Q, int64_t n, int dims, int levels) {
+    const int64_t m2 = levels > 0 && m >= 2 ? ((m - 1) << levels) + 1 : m;
+    const int64_t n2 = levels > 0 && n >= 2 ? ((n - 1) << levels) + 1 : n;
+    std::vector<double> Pr((size_t)(m2 * dims)), Qr((size_t)(n2 * dims));
+    or_refine(P, m, dims, levels, Pr.data());
+    or_refine(Q, n, dims, levels, Qr.data());
+    return dfd_impl<double>(Pr.data(), m2, Qr.data(), n2, dims);
+}
+
 double or_dfd(const double *P, int64_t m, const double *Q, int64_t n, int dims) {
     return dfd_impl<double>(P, m, Q, n, dims);
 }
@@ -471,16 +520,52 @@ struct BlockJob {
     const Progress *ready_in;  // columns of edge_in already published
     double *edge_out;
     Progress *ready_out;
-    double *last;              // DFD: top of the block's last cell
+    double *last;              // DFD: top of the block's last cell; VE-FD: its min(T, R);
+                               // decision: its rightUB
+    const double *A4b = nullptr, *inv2b = nullptr;  // decision: per-column terms
 };
 
 // One row of one column chunk.  `left` is read from lin (the row above), the
 // row's outputs go to lout (lin == lout is allowed: each column is read
 // before it is written).
 inline double row_chunk(const BlockJob &J, int64_t i, int64_t c0, int64_t c1, double bt,
-                        const double *lin, double *lout) {
+                        const double *lin, double *lout, double *lastv = nullptr) {
     const int D = J.D;
     const double *pa = J.P + i * D, *pb = pa + D;
+    if (J.kind == 1) {
+        // decide_rows (reachable<D>, .cu:1173-1244).  A cell whose two inputs
+        // are unreachable (< 0) gives (-1, -2, -1) whatever its intervals (the
+        // rules below yield exactly that), so its intervals are not computed.
+        const double r2 = J.r * J.r;
+        const double Aa = seg_len2(pa, pb, D);
+        const double A4a = 4.0 * Aa, inv2a = 0.5 / Aa;
+        double rUBl = -2.0;
+        for (int64_t j = c0; j < c1; ++j) {
+            const double leftLB = lin[j - c0], botLB = bt;
+            double rLB = -1.0, rUB = -2.0, tLB = -1.0, tUB = -2.0;
+            if (!(leftLB < 0.0 && botLB < 0.0)) {
+                const double *qa = J.Q + j * D, *qb = qa + D;
+                free_iv(pb, qa, qb, D, r2, J.A4b[j], J.inv2b[j], rLB, rUB);
+                free_iv(qb, pa, pb, D, r2, A4a, inv2a, tLB, tUB);
+                if (botLB < 0.0) {
+                    const bool inv = leftLB > rUB || leftLB < 0.0;
+                    const double t2 = std::fmax(rLB, leftLB);
+                    rLB = inv ? -1.0 : t2;
+                    rUB = inv ? -2.0 : rUB;
+                }
+                if (leftLB < 0.0) {
+                    const bool inv = botLB > tUB || botLB < 0.0;
+                    const double t2 = std::fmax(tLB, botLB);
+                    tLB = inv ? -1.0 : t2;
+                }
+            }
+            bt = tLB;
+            lout[j - c0] = rLB;
+            rUBl = rUB;
+        }
+        if (lastv) *lastv = rUBl;
+        return bt;
+    }
     if (J.kind == 0) {
         for (int64_t j = c0; j < c1; ++j) {
             const double *qa = J.Q + j * D, *qb = qa + D;
@@ -490,6 +575,7 @@ inline double row_chunk(const BlockJob &J, int64_t i, int64_t c0, int64_t c1, do
             bt = std::fmax(wt, in);
             lout[j - c0] = std::fmax(wr, in);
         }
+        if (lastv && c1 > c0) *lastv = std::fmin(bt, lout[c1 - 1 - c0]);
     } else {
         for (int64_t j = c0; j < c1; ++j) {
             const double d = dist2(pa, J.Q + j * D, D);
@@ -509,9 +595,14 @@ void run_block(BlockJob J) {
     }
     const int64_t rows = J.b - J.a;
     std::vector<double> bot((size_t)rows);
-    for (int64_t i = J.a; i < J.b; ++i)
-        bot[(size_t)(i - J.a)] = (J.kind == 0 && i == 0) ? J.r * J.r : J.big;
+    for (int64_t i = J.a; i < J.b; ++i) {
+        double v = J.big;                          // DFD, VE-FD rows after the first
+        if (J.kind == 0 && i == 0) v = J.r * J.r;  // VE-FD row 0 (.cu:258-259)
+        if (J.kind == 1) v = (i == 0) ? 0.0 : -1.0;  // decision (.cu:1026-1027)
+        bot[(size_t)(i - J.a)] = v;
+    }
     std::vector<double> buf((size_t)J.chunk), first((size_t)J.chunk);
+    double lastv = 0.0;
     for (int64_t c0 = 0; c0 < J.cols; c0 += J.chunk) {
         const int64_t c1 = std::min(J.cols, c0 + J.chunk);
         if (J.edge_in) {
@@ -519,17 +610,21 @@ void run_block(BlockJob J) {
             std::copy(J.edge_in + c0, J.edge_in + c1, first.begin());
         } else {
             // vefd_rows: the first row enters with left = big; dfd_impl on
-            // P + r0: (r0, 0) enters with 0, the rest of the row with big.
-            std::fill(first.begin(), first.begin() + (c1 - c0), J.big);
+            // P + r0: (r0, 0) enters with 0, the rest of the row with big;
+            // decide_rows: the left boundary is -1 (.cu:656).
+            std::fill(first.begin(), first.begin() + (c1 - c0), J.kind == 1 ? -1.0 : J.big);
             if (J.kind == 2 && c0 == 0) first[0] = 0.0;
         }
-        bot[0] = row_chunk(J, J.a, c0, c1, bot[0], first.data(), buf.data());
-        for (int64_t i = J.a + 1; i < J.b; ++i)
-            bot[(size_t)(i - J.a)] = row_chunk(J, i, c0, c1, bot[(size_t)(i - J.a)], buf.data(), buf.data());
+        double *lv = (c1 == J.cols && rows == 1) ? &lastv : nullptr;
+        bot[0] = row_chunk(J, J.a, c0, c1, bot[0], first.data(), buf.data(), lv);
+        for (int64_t i = J.a + 1; i < J.b; ++i) {
+            lv = (c1 == J.cols && i + 1 == J.b) ? &lastv : nullptr;
+            bot[(size_t)(i - J.a)] = row_chunk(J, i, c0, c1, bot[(size_t)(i - J.a)], buf.data(), buf.data(), lv);
+        }
         std::copy(buf.begin(), buf.begin() + (c1 - c0), J.edge_out + c0);
         J.ready_out->v.store(c1, std::memory_order_release);
     }
-    *J.last = bot[(size_t)rows - 1];
+    *J.last = (J.kind == 2) ? bot[(size_t)rows - 1] : lastv;
 }
 
 }  // namespace
@@ -565,6 +660,40 @@ double or_sweep_rows_mt(const double *P, int64_t m, const double *Q, int64_t n, 
     double s = 0;
     for (double v : edge[(size_t)threads - 1]) s += v;
     return s;
+}
+
+double or_sweep_mt(const double *P, int64_t m, const double *Q, int64_t n, int dims, int kind, double r,
+                   int threads) {
+    const int64_t rows = (kind == 2) ? m : m - 1;
+    const int64_t cols = (kind == 2) ? n : n - 1;
+    if (rows < 1 || cols < 1 || kind < 0 || kind > 2) return std::nan("");
+    if (threads < 1) threads = 1;
+    if (threads > rows) threads = (int)rows;
+    const int64_t B = (rows + threads - 1) / threads;
+    threads = (int)((rows + B - 1) / B);
+    std::vector<double> A4b, inv2b;
+    if (kind == 1) q_seg_terms(Q, n, dims, A4b, inv2b);
+    std::vector<std::vector<double>> edge((size_t)threads, std::vector<double>((size_t)cols));
+    std::vector<Progress> ready((size_t)threads);
+    std::vector<double> last((size_t)threads * 16, 0.0);
+    std::vector<BlockJob> jobs((size_t)threads);
+    for (int t = 0; t < threads; ++t) {
+        BlockJob &J = jobs[(size_t)t];
+        J.P = P; J.Q = Q; J.D = dims; J.kind = kind; J.r = r; J.big = (double)FLT_MAX;
+        J.a = t * B; J.b = std::min(rows, J.a + B); J.cols = cols; J.chunk = 256;
+        J.edge_in = t > 0 ? edge[(size_t)t - 1].data() : nullptr;
+        J.ready_in = t > 0 ? &ready[(size_t)t - 1] : nullptr;
+        J.edge_out = edge[(size_t)t].data();
+        J.ready_out = &ready[(size_t)t];
+        J.last = &last[(size_t)t * 16];
+        if (kind == 1) { J.A4b = A4b.data(); J.inv2b = inv2b.data(); }
+    }
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(run_block, jobs[(size_t)t]);
+    run_block(jobs[0]);
+    for (auto &th : pool) th.join();
+    const double v = last[(size_t)(threads - 1) * 16];
+    return kind == 1 ? v : std::sqrt(v);
 }
 
 }  // extern "C"

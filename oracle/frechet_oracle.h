@@ -27,6 +27,14 @@ double or_endpoint_lb(const double *P, int64_t m, const double *Q, int64_t n, in
 /* DFD<D> (.cu:458-641): discrete Frechet distance on squared distances. */
 double or_dfd(const double *P, int64_t m, const double *Q, int64_t n, int dims);
 
+/* A tighter upper bound than DFD, an addition beside Bound_call_gpu
+ * (.cu:1262-1326): DFD of both curves with every segment split at its
+ * midpoint, `levels` times (or_refine: point a + 0.5 (b - a) between
+ * consecutive points a, b; returns the refined length, writes it to out when
+ * out != NULL).  FD <= or_dfd_refined <= DFD. */
+int64_t or_refine(const double *P, int64_t m, int dims, int levels, double *out);
+double or_dfd_refined(const double *P, int64_t m, const double *Q, int64_t n, int dims, int levels);
+
 /* VEFD<D> (.cu:200-432, README.md:77-126): vertex-edge Frechet bound.
  * init_lb is the endpoint bound; cell (0,0) enters with init_lb*init_lb. */
 double or_vefd(const double *P, int64_t m, const double *Q, int64_t n, int dims,
@@ -38,7 +46,9 @@ double or_decide_ub(const double *P, int64_t m, const double *Q, int64_t n, int 
                     double r);
 int or_decide(const double *P, int64_t m, const double *Q, int64_t n, int dims, double r);
 
-/* Same formulas evaluated in FP32 (points rounded to float, r rounded to float). */
+/* Same formulas evaluated in FP32 (points rounded to float, r rounded to float);
+ * in the decision, the square root of the discriminant and the two interval
+ * ends are computed in FP64 from the FP32 terms and rounded to FP32 once. */
 double or_vefd_f32(const double *P, int64_t m, const double *Q, int64_t n, int dims,
                    double init_lb);
 double or_decide_ub_f32(const double *P, int64_t m, const double *Q, int64_t n, int dims,
@@ -92,6 +102,16 @@ double or_sweep_rows(const double *P, int64_t m, const double *Q, int64_t n, int
  * chunk), for the multi-core CPU timing.  Bit-identical to or_sweep_rows. */
 double or_sweep_rows_mt(const double *P, int64_t m, const double *Q, int64_t n, int dims,
                         int kind, double r, int64_t r0, int64_t r1, int threads);
+
+/* The terminal value of one whole sweep on `threads` CPU threads, with the
+ * row-block pipeline of or_sweep_rows_mt: kind 0 = VE-FD (r = the endpoint
+ * bound, as or_vefd), 1 = decision at radius r (terminal rightUB, as
+ * or_decide_ub; cells whose inputs are both unreachable skip their interval
+ * arithmetic, whose outputs there are (-1, -2, -1) either way), 2 = DFD (as
+ * or_dfd).  Bit-identical to the single-thread functions; used to compute the
+ * reference values of long pairs. */
+double or_sweep_mt(const double *P, int64_t m, const double *Q, int64_t n, int dims, int kind, double r,
+                   int threads);
 
 #ifdef __cplusplus
 }

@@ -61,6 +61,13 @@ def lib():
         L.or_sweep_rows_mt.argtypes = [_dp, _i64, _dp, _i64, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_double, _i64, _i64, ctypes.c_int]
         L.or_sweep_rows_mt.restype = ctypes.c_double
+        L.or_sweep_mt.argtypes = [_dp, _i64, _dp, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_int]
+        L.or_sweep_mt.restype = ctypes.c_double
+        L.or_refine.argtypes = [_dp, _i64, ctypes.c_int, ctypes.c_int, _dp]
+        L.or_refine.restype = _i64
+        L.or_dfd_refined.argtypes = [_dp, _i64, _dp, _i64, ctypes.c_int, ctypes.c_int]
+        L.or_dfd_refined.restype = ctypes.c_double
         _LIB = L
     return _LIB
 
@@ -167,3 +174,26 @@ def sweep_rows_mt(P, Q, kind: int, r: float, r0: int, r1: int, threads: int) -> 
     P, Q = _c(P), _c(Q)
     return lib().or_sweep_rows_mt(_p(P), len(P), _p(Q), len(Q), P.shape[1], kind, r, r0, r1,
                                   threads)
+
+
+def sweep_mt(P, Q, kind: int, r: float, threads: int) -> float:
+    """Terminal value of a whole sweep on `threads` CPU threads: kind 0 VE-FD
+    (r = endpoint bound; same as vefd), 1 decision at r (terminal rightUB;
+    same as decide_ub), 2 DFD (same as dfd)."""
+    P, Q = _c(P), _c(Q)
+    return lib().or_sweep_mt(_p(P), len(P), _p(Q), len(Q), P.shape[1], kind, float(r), int(threads))
+
+
+def refine(P, levels: int) -> np.ndarray:
+    """The curve with every segment split at its midpoint, `levels` times."""
+    P = _c(P)
+    n = lib().or_refine(_p(P), len(P), P.shape[1], levels, None)
+    out = np.zeros((n, P.shape[1]))
+    lib().or_refine(_p(P), len(P), P.shape[1], levels, _p(out))
+    return out
+
+
+def dfd_refined(P, Q, levels: int) -> float:
+    """DFD of the midpoint-refined curves: an upper bound between FD and DFD."""
+    P, Q = _c(P), _c(Q)
+    return lib().or_dfd_refined(_p(P), len(P), _p(Q), len(Q), P.shape[1], levels)

@@ -36,7 +36,7 @@ class Opts(ctypes.Structure):
                 ("verbose", ctypes.c_int32), ("mixed_switch", ctypes.c_double),
                 ("rows_per_lane", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("prune", ctypes.c_int32), ("spec_depth", ctypes.c_int32),
-                ("decide_form", ctypes.c_int32)]
+                ("decide_form", ctypes.c_int32), ("refine_ub", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):
@@ -53,12 +53,14 @@ class Result(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [("launches", ctypes.c_int64 * N_KINDS), ("ms", ctypes.c_double * N_KINDS),
-                ("cells", ctypes.c_double * N_KINDS), ("evaluated", ctypes.c_double * N_KINDS)]
+                ("cells", ctypes.c_double * N_KINDS), ("evaluated", ctypes.c_double * N_KINDS),
+                ("skipped_blocks", ctypes.c_double * N_KINDS)]
 
     def as_dict(self) -> dict:
         return {KERNEL_NAMES[k]: dict(launches=int(self.launches[k]), ms=float(self.ms[k]),
                                       cells=float(self.cells[k]),
-                                      evaluated=float(self.evaluated[k]))
+                                      evaluated=float(self.evaluated[k]),
+                                      skipped_blocks=float(self.skipped_blocks[k]))
                 for k in range(N_KINDS) if self.launches[k]}
 
 
@@ -82,6 +84,7 @@ SIGNATURES = {
     "rfd_session_bounds": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, ctypes.c_int]),
     "rfd_session_decide": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _ip, _dp]),
     "rfd_session_set_decide_form": (ctypes.c_int, [_vp, ctypes.c_int32, _ip]),
+    "rfd_session_refined_ub": (ctypes.c_int, [_vp, ctypes.c_int32, _dp]),
     "rfd_session_distance": (ctypes.c_int, [_vp, ctypes.POINTER(Opts), ctypes.POINTER(Result)]),
     "rfd_distance": (ctypes.c_int, [_vp, _dp, _i64, _dp, _i64, ctypes.c_int,
                                     ctypes.POINTER(Opts), ctypes.POINTER(Result)]),

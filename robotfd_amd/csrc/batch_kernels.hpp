@@ -180,7 +180,7 @@ __global__ __launch_bounds__(64) void batch_sweep(BatchArgs a) {
     __shared__ double qring[4][3][64];
     __shared__ double sring[4][Staged<OP>::value][64];
     const int lane = (int)threadIdx.x;
-    unsigned long long evals = 0;
+    unsigned long long evals = 0, skips = 0;
     const int total = __hip_atomic_load(&a.count[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
         int idx = 0;
@@ -210,9 +210,9 @@ __global__ __launch_bounds__(64) void batch_sweep(BatchArgs a) {
         job.watch = a.watch;
         // A decision item runs the interval form of its pair (wave-uniform).
         if (OP == OP_DEC && p.dec_form == DEC_TWO_PHASE)
-            run_band<OP, 3, double, SYNC_SCOPED, R, true>(job, band, qring, sring, evals, nullptr);
+            run_band<OP, 3, double, SYNC_SCOPED, R, true>(job, band, qring, sring, evals, nullptr, skips);
         else
-            run_band<OP, 3, double, SYNC_SCOPED, R, PRUNE>(job, band, qring, sring, evals, nullptr);
+            run_band<OP, 3, double, SYNC_SCOPED, R, PRUNE>(job, band, qring, sring, evals, nullptr, skips);
     }
     if (a.evals) atomicAdd(a.evals, evals);
 }

@@ -510,7 +510,7 @@ static rfd_status batch_run_impl(rfd_batch *b, const rfd_opts *opts, rfd_result 
         r.vefd = st.vefd;
         r.lb = st.lb;
         r.ub = st.ub;
-        r.err_bound = st.distance > 0 ? (st.ub - st.lb) / 2.0 / st.distance : 0.0;
+        r.err_bound = rfd_host::error_bound(st.lb, st.ub, st.distance);
         r.iterations = st.iterations;
         r.ms_total = st.t_done >= t0 ? (double)(st.t_done - t0) / ctx->wall_khz : 0.0;
         if (latency_ms) latency_ms[k] = r.ms_total;

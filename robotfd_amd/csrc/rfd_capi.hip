@@ -54,7 +54,8 @@ __global__ void sweep_init(int *progress, int nbands, int *queue, double *result
     if (tid == 0) {
         *queue = 0;
         for (int k = 0; k < nres; ++k) result[k] = sentinel;
-        *evals = 0;
+        evals[0] = 0;
+        evals[1] = 0;
         *abort = 0;
     }
 }
@@ -62,6 +63,16 @@ __global__ void sweep_init(int *progress, int nbands, int *queue, double *result
 __global__ void to_f32(const double *src, float *dst, long long count) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < count) dst[k] = (float)src[k];
+}
+
+// One level of midpoint refinement: dst gets src's points with a + 0.5 (b - a)
+// between consecutive points a, b (2 len - 1 points).
+__global__ void refine_level(const double *src, long long len, int dims, double *dst) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (2 * len - 1) * dims) return;
+    const long long pt = k / dims, c = k % dims, i = pt >> 1;
+    const double a = src[i * dims + c];
+    dst[k] = (pt & 1) ? a + 0.5 * (src[(i + 1) * dims + c] - a) : a;
 }
 
 // flag[0] <- 0 (a kernel, not a memset, so that the call stays capturable).
@@ -264,10 +275,10 @@ rfd_status run_sweep(rfd_session *s, const T *P, const T *Q, const T *qseg, doub
     RFD_HIP(ctx, (launch_sweep<OP, T>(s->dims, grid, ctx->sync, rpl, ctx->stream, a)));
     RFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     double hres[2] = {0, 0};
-    unsigned long long hev = 0;
+    unsigned long long hev[2] = {0, 0};
     int habort = 0;
     RFD_HIP(ctx, hipMemcpyAsync(hres, s->result, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    RFD_HIP(ctx, hipMemcpyAsync(&hev, s->evals, sizeof(hev), hipMemcpyDeviceToHost, ctx->stream));
+    RFD_HIP(ctx, hipMemcpyAsync(hev, s->evals, sizeof(hev), hipMemcpyDeviceToHost, ctx->stream));
     RFD_HIP(ctx, hipMemcpyAsync(&habort, s->abort, sizeof(habort), hipMemcpyDeviceToHost, ctx->stream));
     RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const double h = hres[0];
@@ -277,7 +288,8 @@ rfd_status run_sweep(rfd_session *s, const T *P, const T *Q, const T *qseg, doub
     ctx->stats.launches[kind] += 1;
     ctx->stats.ms[kind] += ms;
     ctx->stats.cells[kind] += (double)rows * (double)cols;
-    ctx->stats.evaluated[kind] += (double)hev;
+    ctx->stats.evaluated[kind] += (double)hev[0];
+    ctx->stats.skipped_blocks[kind] += (double)hev[1];
     if (tlog) {
         // Debug instrumentation: header {kind, rows, cols, nbands, grid, sync} + 4 u64 per band.
         std::vector<unsigned long long> h_t((size_t)(4 * nb));
@@ -395,6 +407,66 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
     return RFD_OK;
 }
 
+// Midpoint refinement of a device curve, `levels` times, into a new device
+// array (*out, *len_out); the caller frees it.
+rfd_status refine_curve(rfd_ctx *ctx, const double *src, int64_t len, int dims, int levels, double **out,
+                        int64_t *len_out) {
+    int64_t cur_len = len;
+    double *cur = nullptr;
+    RFD_HIP(ctx, hipMalloc(&cur, sizeof(double) * len * dims));
+    RFD_HIP(ctx, hipMemcpyAsync(cur, src, sizeof(double) * len * dims, hipMemcpyDeviceToDevice, ctx->stream));
+    for (int l = 0; l < levels && cur_len >= 2; ++l) {
+        const int64_t nl = 2 * cur_len - 1;
+        double *nxt = nullptr;
+        if (hipMalloc(&nxt, sizeof(double) * nl * dims) != hipSuccess) {
+            (void)hipFree(cur);
+            set_err(ctx, "hipMalloc(refined curve) failed");
+            return RFD_ERR_NOMEM;
+        }
+        const long long cnt = nl * dims;
+        hipLaunchKernelGGL(rfd::refine_level, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (const double *)cur, (long long)cur_len, dims, nxt);
+        RFD_HIP(ctx, hipGetLastError());
+        RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        (void)hipFree(cur);
+        cur = nxt;
+        cur_len = nl;
+    }
+    *out = cur;
+    *len_out = cur_len;
+    return RFD_OK;
+}
+
+// DFD of both curves of a session refined `levels` times (device buffers of
+// the refined sizes for the sweep; the session's small words are reused).
+rfd_status refined_ub(rfd_session *s, int levels, double *ub) {
+    rfd_ctx *ctx = s->ctx;
+    rfd_session t;
+    t.ctx = ctx;
+    t.dims = s->dims;
+    t.queue = s->queue;
+    t.result = s->result;
+    t.evals = s->evals;
+    t.abort = s->abort;
+    rfd_status st = refine_curve(ctx, s->dP, s->m, s->dims, levels, &t.dP, &t.m);
+    if (st == RFD_OK) st = refine_curve(ctx, s->dQ, s->n, s->dims, levels, &t.dQ, &t.n);
+    if (st == RFD_OK && hipMalloc(&t.slots, sizeof(double) * 2 * t.n) != hipSuccess) st = RFD_ERR_NOMEM;
+    if (st == RFD_OK && hipMalloc(&t.progress, sizeof(int) * (nbands_for(t.m) + 1)) != hipSuccess) st = RFD_ERR_NOMEM;
+    if (st == RFD_OK) {
+        double F = 0;
+        st = run_sweep<rfd::OP_DFD, double>(&t, t.dP, t.dQ, nullptr, 0.0, RFD_K_DFD, ub, INFINITY, &F);
+    } else if (st == RFD_ERR_NOMEM) {
+        set_err(ctx, "no device memory for the refined curves (%lld x %lld points)", (long long)t.m,
+                (long long)t.n);
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(t.dP);
+    (void)hipFree(t.dQ);
+    (void)hipFree(t.slots);
+    (void)hipFree(t.progress);
+    return st;
+}
+
 bool keep_bisecting(double lb, double ub, const rfd_opts &o, double eps) {
     return o.stop_mode == RFD_STOP_ABS ? (ub - lb > eps) : (ub - lb > eps * ub);
 }
@@ -456,6 +528,7 @@ void rfd_default_opts(rfd_opts *o) {
     o->use_graph = 1;
     o->prune = -1;
     o->spec_depth = 0;
+    o->refine_ub = 0;
 }
 
 rfd_status rfd_create(rfd_ctx **out, int device) {
@@ -603,7 +676,7 @@ rfd_status rfd_session_create(rfd_ctx *ctx, const double *P, int64_t m, const do
         return fail(e, "hipMalloc(progress)");
     if ((e = hipMalloc(&s->queue, sizeof(int))) != hipSuccess) return fail(e, "hipMalloc(queue)");
     if ((e = hipMalloc(&s->result, 4 * sizeof(double))) != hipSuccess) return fail(e, "hipMalloc(result)");
-    if ((e = hipMalloc(&s->evals, sizeof(unsigned long long))) != hipSuccess) return fail(e, "hipMalloc(evals)");
+    if ((e = hipMalloc(&s->evals, 2 * sizeof(unsigned long long))) != hipSuccess) return fail(e, "hipMalloc(evals)");
     if ((e = hipMalloc(&s->abort, sizeof(int))) != hipSuccess) return fail(e, "hipMalloc(abort)");
     if ((e = hipMemcpyAsync(s->dP, P, sizeof(double) * m * dims, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
         return fail(e, "hipMemcpy(P)");
@@ -683,6 +756,16 @@ rfd_status rfd_session_decide(rfd_session *s, double radius, int fp32, int *reac
 }
 
 
+rfd_status rfd_session_refined_ub(rfd_session *s, int32_t levels, double *ub) {
+    if (!s || !ub || levels < 0 || levels > 8) return RFD_ERR_ARG;
+    RFD_HIP(s->ctx, hipSetDevice(s->ctx->device));
+    if (levels == 0 || s->m < 2 || s->n < 2) {
+        double F = 0;
+        return run_sweep<rfd::OP_DFD, double>(s, s->dP, s->dQ, nullptr, 0.0, RFD_K_DFD, ub, INFINITY, &F);
+    }
+    return refined_ub(s, levels, ub);
+}
+
 rfd_status rfd_session_set_decide_form(rfd_session *s, int32_t form, int32_t *effective) {
     if (!s || form < RFD_DECIDE_FUSED || form > RFD_DECIDE_REFERENCE) return RFD_ERR_ARG;
     RFD_HIP(s->ctx, hipSetDevice(s->ctx->device));
@@ -729,6 +812,14 @@ static rfd_status session_distance_impl(rfd_session *s, const rfd_opts *opts, rf
     res->vefd = lb;
     if (o.verbose) std::printf("LB: %.14f\nUB: %.14f\n", lb, ub);  // frechet.cpp:34
     const double eps = o.eps;
+    if (o.refine_ub > 0 && !degenerate && keep_bisecting(lb, ub, o, eps)) {
+        // The tighter upper bound (opt-in): DFD of the midpoint-refined curves,
+        // at most DFD and, up to the midpoints' rounding, at least FD.
+        double rub = ub;
+        if ((st = refined_ub(s, o.refine_ub, &rub)) != RFD_OK) return st;
+        ub = std::max(lb, std::min(ub, rub));
+        if (o.verbose) std::printf("Refined UB: %.14f\n", ub);
+    }
     int64_t iter = 0, iter32 = 0, cert = 0;
     if (mixed && !degenerate) {
         // FP32 VE-FD is an estimate, not a certified bound: clamp to [endpoint, DFD].
@@ -810,7 +901,7 @@ static rfd_status session_distance_impl(rfd_session *s, const rfd_opts *opts, rf
     res->distance = value;
     res->lb = lb;
     res->ub = ub;
-    res->err_bound = value > 0 ? (ub - lb) / 2.0 / value : 0.0;
+    res->err_bound = rfd_host::error_bound(lb, ub, value);
     res->iterations = iter;
     res->iterations_fp32 = iter32;
     res->certify_steps = cert;
@@ -865,6 +956,10 @@ rfd_status rfd_distance_noalloc(rfd_ctx *ctx, const double *dP, int64_t m, const
     }
     if (o.precision != RFD_PREC_FP64) {
         set_err(ctx, "rfd_distance_noalloc computes in FP64 only (precision must be RFD_PREC_FP64)");
+        return RFD_ERR_ARG;
+    }
+    if (o.refine_ub != 0) {
+        set_err(ctx, "rfd_distance_noalloc does not refine the upper bound (refine_ub must be 0)");
         return RFD_ERR_ARG;
     }
     if (o.decide_form < RFD_DECIDE_FUSED || o.decide_form > RFD_DECIDE_REFERENCE) return RFD_ERR_ARG;

@@ -96,6 +96,15 @@ inline double endpoint_bound(const double *P, int64_t m, const double *Q, int64_
 }
 
 
+// Relative error bound of a result v = (lb + ub) / 2 for a distance FD in
+// [lb, ub]: |v - FD| / FD <= max(ub - v, v - lb) / lb.  The differences are
+// exact (Sterbenz) and account for the rounding of v itself; 0 when the
+// bracket is a point.
+inline double error_bound(double lb, double ub, double v) {
+    const double w = std::max(ub - v, v - lb);
+    return w > 0.0 ? (lb > 0.0 ? w / lb : INFINITY) : 0.0;
+}
+
 // Rejects curves holding a NaN or infinite coordinate: the sweeps' min / max
 // recursions and the fused bound sweep (compiled without NaN semantics) assume
 // finite points.  pair >= 0 names a batch pair in the message.

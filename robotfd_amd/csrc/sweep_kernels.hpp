@@ -170,16 +170,24 @@ __device__ inline void iv_disc(const T *c, const T *s0, const T *s1, T r2, T A4,
     disc = N::fma(B, B, -(A4 * C));
 }
 
+// The square root and the interval ends are FP64 in both precisions: in FP32
+// (the precision form's early decisions) they are computed in FP64 from the
+// FP32 B, disc and inv2A, and the ends rounded to FP32 once.
+template <typename T> struct RootT { using type = T; };
+template <> struct RootT<float> { using type = double; };
+
 template <typename T>
-__device__ inline T iv_root(T disc) {
-    return Num<T>::sqrt(disc >= T(0) ? disc : T(0));
+__device__ inline typename RootT<T>::type iv_root(T disc) {
+    using R = typename RootT<T>::type;
+    return Num<R>::sqrt(disc >= T(0) ? (R)disc : R(0));
 }
 
 template <typename T, bool TWO_PHASE>
-__device__ inline void iv_ends(T B, T disc, T s, T inv2A, T &lb, T &ub) {
+__device__ inline void iv_ends(T B, T disc, typename RootT<T>::type s, T inv2A, T &lb, T &ub) {
     using N = Num<T>;
-    const T ta = (-B - s) * inv2A;
-    const T tb = (-B + s) * inv2A;
+    using R = typename RootT<T>::type;
+    const T ta = (T)((-(R)B - s) * (R)inv2A);
+    const T tb = (T)((-(R)B + s) * (R)inv2A);
     // Bitwise tests: with && the compiler sinks the root and the ends into a
     // branch on disc >= 0, which keeps the cells' chains from interleaving.
     const bool valid = (disc >= T(0)) & !((tb < T(0)) | (ta > T(1)));
@@ -228,7 +236,8 @@ struct SweepArgs {
     int nbands;
     double param;        // VE-FD: endpoint bound; decision: radius
     double thr;          // VE-FD: prune threshold on incoming values (+inf: none)
-    unsigned long long *evals;  // cells whose weights / free intervals were evaluated
+    unsigned long long *evals;  // [0] cells whose weights / free intervals were evaluated,
+                                //  [1] whole dead blocks skipped
     unsigned long long *tlog;   // optional, 4 per band: wave start, first input ready, end, hw id
     int dec_form;        // decision: DEC_FUSED or DEC_TWO_PHASE (selects the kernel variant)
     Watch watch;         // bound on the hand-off waits
@@ -319,7 +328,7 @@ template <int OP> struct Staged { static constexpr int value = (OP == OP_DEC) ? 
 template <int OP, int D, typename T, int SYNC, int R, bool VAR = false>
 __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4][D][64],
                                 T (&sring)[4][Staged<OP>::value][64], unsigned long long &evals,
-                                unsigned long long *tlog) {
+                                unsigned long long *tlog, unsigned long long &skips) {
     using N = Num<T>;
     constexpr int NS = (OP == OP_DEC) ? 2 : 0;  // per-column terms staged
     constexpr int NP = (OP == OP_DFD) ? R : R + 1;  // P points a lane holds
@@ -424,7 +433,8 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
         }
         const T A4b = sring[ra][0][oa];
         const T inv2b = sring[ra][NS - 1][oa];
-        T Bv[2 * R], Dv[2 * R], Sv[2 * R];
+        T Bv[2 * R], Dv[2 * R];
+        typename RootT<T>::type Sv[2 * R];
 #pragma unroll
         for (int t = 0; t < R; ++t) {
             iv_disc<T, D, VAR>(p[t + 1], qa, qb, r2, A4b, Bv[2 * t], Dv[2 * t]);
@@ -501,6 +511,7 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
                 }
             }
             if (skip) {
+                ++skips;  // whole dead blocks skipped (wave-uniform)
                 if (has_succ) {
                     // Columns s-64 (lane s mod 64 of `bout`, output at step
                     // s-1) .. s-65+kHand form a full chunk.
@@ -547,8 +558,9 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
                 // Every lane propagates (no branch, so the next step's
                 // intervals above and this chain interleave); lanes outside
                 // the grid keep their state.  Rows past the last (padding of
-                // the last band) are propagated too: their values only reach
-                // padding rows above them, and the last band has no successor.
+                // the last band) are propagated too, but keep their state and
+                // hand -1 upward: they stay dead, so whole dead blocks of the
+                // last band can still be skipped.
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
                     const long long it = i0 + t;
@@ -570,8 +582,8 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
                     const T rLB = bdead ? (rinv ? T(-1) : rmax) : rl[t];
                     const T rUB = (bdead & rinv) ? T(-2) : ru[t];
                     const T tLB = ldead ? (tinv ? T(-1) : tmax) : tl[t];
-                    bot[t] = inr ? tLB : bot[t];
-                    left = rLB;
+                    bot[t] = (inr && it < rows) ? tLB : bot[t];
+                    left = (it < rows) ? rLB : T(-1);
                     term = (inr && it == rows - 1 && j == cols - 1) ? rUB : term;
                 }
                 out = inr ? left : out;
@@ -679,15 +691,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     const int lane = (int)threadIdx.x;
     SweepJob job{a.P, a.Q, a.qseg, a.slots, a.progress, a.result, a.rows, a.cols, a.nbands,
                  a.param, a.thr, a.watch};
-    unsigned long long evals = 0;
+    unsigned long long evals = 0, skips = 0;
     for (;;) {
         int band = 0;
         if (lane == 0) band = atomicAdd(a.queue, 1);
         band = __builtin_amdgcn_readfirstlane(band);
         if (band >= a.nbands) break;
-        run_band<OP, D, T, SYNC, R, VAR>(job, band, qring, sring, evals, a.tlog);
+        run_band<OP, D, T, SYNC, R, VAR>(job, band, qring, sring, evals, a.tlog, skips);
     }
-    if (a.evals) atomicAdd(a.evals, evals);
+    if (a.evals) {
+        atomicAdd(a.evals, evals);
+        if (lane == 0) atomicAdd(a.evals + 1, skips);
+    }
 }
 
 // Per-column terms of the decision sweep (.cu:1129-1144): 4*Ab and 0.5/Ab.

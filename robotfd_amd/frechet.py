@@ -137,7 +137,7 @@ class Context:
 
 def _opts(stop: str = "abs", eps: float = FLT_EPSILON, precision: str = "fp64",
           vefd_gate: bool = True, verbose: bool = False, mixed_switch: float = 1e-4,
-          decide_form: str = "fused") -> _lib.Opts:
+          decide_form: str = "fused", refine_ub: int = 0) -> _lib.Opts:
     """decide_form: "fused" (reachable<D>'s intervals), "two-phase"
     (interval_calculation<D>'s) or "reference" (the reference's choice from
     free device memory, per pair)."""
@@ -150,6 +150,7 @@ def _opts(stop: str = "abs", eps: float = FLT_EPSILON, precision: str = "fp64",
     o.vefd_gate = 1 if vefd_gate else 0
     o.verbose = 1 if verbose else 0
     o.mixed_switch = float(mixed_switch)
+    o.refine_ub = int(refine_ub)
     return o
 
 
@@ -211,6 +212,14 @@ class Intersection:
         check(lib().rfd_session_bounds(self.handle, ctypes.byref(ub), ctypes.byref(ve),
                                        1 if run_vefd else 0, 1 if fp32 else 0), self.ctx.handle)
         return ub.value, ve.value
+
+    def refined_ub(self, levels: int = 1) -> float:
+        """DFD of both curves refined `levels` times at segment midpoints: an
+        upper bound of the distance at most DFD (rfd_session_refined_ub)."""
+        self._need()
+        ub = ctypes.c_double(0.0)
+        check(lib().rfd_session_refined_ub(self.handle, int(levels), ctypes.byref(ub)), self.ctx.handle)
+        return ub.value
 
     def intersection_interval_call_gpu(self, radius: float, fp32: bool = False) -> bool:
         return self.decide_ub(radius, fp32) == 1.0

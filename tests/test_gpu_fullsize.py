@@ -1,0 +1,119 @@
+"""Parity at the full sizes of the benchmark workloads, against reference
+values the CPU oracle computed (tests/golden/make_fullsize_golden.py):
+
+* the 100,000 x 120,000 noisy-copy pair and the 200,000 x 200,000 random-walk
+  pair: DFD and VE-FD of the fused bound sweep and of the separate band sweeps
+  (at 200k the latter run 1,563 bands, more rows than any wave set holds, the
+  regime of the reference's row wrap, .cu:1039-1056), the terminal rightUB of
+  decisions at DFD (1 -/+ 0.5e-9), the distance and its bisection steps, all
+  bit for bit; and at 200k the mixed-precision form within 1e-9 relative of
+  FP64 with an error bound at least its actual error;
+* the 4,096-pair batch: every pair's DFD, VE-FD, distance and step count bit
+  for bit under plain bisection; speculative bisection identical to plain; the
+  learned bracket (committed weights), with and without speculation, within
+  1e-9 relative of the oracle's distance with lb <= ub for every pair.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from robotfd_amd import Context, Intersection, workloads as W
+from robotfd_amd.batch import Batch
+from robotfd_amd.learned import DEFAULT_WEIGHTS, BracketNet, run_learned
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return json.load(open(os.path.join(GOLDEN, "fullsize.json")))
+
+
+def workload(key):
+    return W.robot_pair(100_000, 120_000, seed=0) if key == "c2" else W.long_3d(200_000, 200_000, seed=0)
+
+
+@pytest.mark.parametrize("key", ["c2", "c3"])
+def test_long_pair_matches_oracle(gpu, gold, key):
+    g = gold[key]
+    P, Q = workload(key)
+    assert [W.sha256(P), W.sha256(Q)] == g["sha256"]
+    say(key, "fused bounds, decisions, distance", P.shape, Q.shape)
+    with Intersection(P, Q, ctx=gpu) as s:
+        dfd, ve = s.Bound_call_gpu(g["endpoint_lb"])
+        assert (dfd, ve) == (g["dfd"], g["vefd"])
+        for r, ub in g["decisions"]:
+            assert s.decide_ub(r) == ub, r
+        d = s.distance(stop="rel", eps=1e-9)
+        assert d["distance"] == g["distance"] and d["iterations"] == g["iterations"]
+        assert (d["dfd"], d["vefd"], d["endpoint_lb"]) == (g["dfd"], g["vefd"], g["endpoint_lb"])
+        if key == "c3":
+            say(key, "mixed precision")
+            mx = s.distance(stop="rel", eps=1e-9, precision="mixed")
+            err = abs(mx["distance"] - g["distance"]) / g["distance"]
+            assert err <= 1e-9
+            assert mx["err_bound"] >= err
+            assert mx["lb"] <= g["distance"] <= mx["ub"]
+
+
+@pytest.mark.parametrize("key", ["c2", "c3"])
+def test_long_pair_separate_band_sweeps_match_oracle(gold, key):
+    """DFD and VE-FD on the separate band sweeps (2 rows per lane: 782 bands
+    at 100k, 1,563 at 200k)."""
+    g = gold[key]
+    P, Q = workload(key)
+    os.environ["RFD_FUSED"] = "0"
+    try:
+        ctx = Context()
+    finally:
+        del os.environ["RFD_FUSED"]
+    try:
+        say(key, "separate band sweeps")
+        with Intersection(P, Q, ctx=ctx) as s:
+            assert s.Bound_call_gpu(g["endpoint_lb"]) == (g["dfd"], g["vefd"])
+    finally:
+        ctx.close()
+
+
+def test_batch_4096_pairs_match_oracle(gpu):
+    z = np.load(os.path.join(GOLDEN, "batch4096.npz"), allow_pickle=False)
+    pairs = W.batch_pairs(4096, seed=0)
+    sizes = np.array([(len(P), len(Q), P.shape[1]) for P, Q in pairs])
+    assert (sizes == z["sizes"]).all()
+    assert W.sha256(np.concatenate([np.concatenate([P.ravel(), Q.ravel()]) for P, Q in pairs])) == \
+        str(z["points_sha256"])
+    open_pairs = z["iterations"] > 0
+    say("batch of 4096 pairs,", int(open_pairs.sum()), "open")
+    with Batch(pairs, ctx=gpu) as b:
+        plain = b.run(rows_per_lane=1, spec_depth=1)
+        for f in ("dfd", "vefd", "endpoint_lb", "distance", "lb", "ub"):
+            bad = np.flatnonzero(plain[f] != z[f])
+            assert bad.size == 0, (f, bad[:5], plain[f][bad[:5]], z[f][bad[:5]])
+        assert (plain["iterations"] == z["iterations"]).all()
+        say("speculative")
+        spec = b.run(rows_per_lane=1, spec_depth=0)
+        for f in ("distance", "lb", "ub", "iterations"):
+            assert (spec[f] == plain[f]).all(), f
+        net = BracketNet(ctx=gpu, path=DEFAULT_WEIGHTS)
+        try:
+            for depth in (1, 0):
+                say("learned, speculation depth", depth)
+                lr = run_learned(b, net, rows_per_lane=1, spec_depth=depth)
+                rel = np.abs(lr["distance"] - z["distance"]) / z["distance"]
+                assert rel.max() <= 1e-9, (depth, int(rel.argmax()), rel.max())
+                assert (lr["lb"] <= lr["ub"]).all()
+                # Both brackets hold the distance, so they overlap.
+                assert (lr["lb"] <= z["ub"]).all() and (lr["ub"] >= z["lb"]).all()
+                assert (lr["dfd"] == z["dfd"]).all() and (lr["vefd"] == z["vefd"]).all()
+        finally:
+            net.close()

@@ -172,13 +172,13 @@ def test_pruned_and_skipped_sweeps_bit_exact(pruning_ctx, kind):
             assert s.decide_ub(r) == O.decide_ub(P, Q, r)
 
 
-@pytest.mark.parametrize("rows", [1, 2, 4])
+@pytest.mark.parametrize("rows", [1, 2, 3, 4])
 def test_decision_dead_skips_across_bands(rows):
     """Decisions below and above the distance on pairs of several bands, at 1,
     2 and 4 rows per lane, where most cells are unreachable: whole 16-step
     blocks can be dead, skipped, and handed to the next band as dead columns.
     The terminal rightUB stays bit-exact; dead cells do occur (fewer cells
-    evaluated than swept)."""
+    evaluated than swept) and whole dead blocks are skipped (counted)."""
     os.environ["RFD_ROWS"] = str(rows)
     try:
         ctx = Context()
@@ -186,18 +186,26 @@ def test_decision_dead_skips_across_bands(rows):
         del os.environ["RFD_ROWS"]
     try:
         rng = np.random.default_rng(40 + rows)
+        skipped = 0
         for d in (2, 3):
             P, Q = walk(rng, 1100, d), walk(rng, 2300, d)
-            lb = O.endpoint_lb(P, Q)
-            dfd = O.dfd(P, Q)
-            say("dead skips", rows, d)
-            with Intersection(P, Q, ctx=ctx) as s:
-                s.data_move()
-                ctx.reset_stats()
-                for f in (0.2, 0.5, 0.8, 0.95, 1.0, 1.05):
-                    r = lb + f * (dfd - lb)
-                    assert s.decide_ub(r) == O.decide_ub(P, Q, r), (d, f)
-                st = ctx.stats()["decide_f64"]
-                assert st["evaluated"] < st["cells"]
+            # A second pair whose endpoints coincide: its endpoint bound is 0,
+            # and at radii well below DFD almost every cell is unreachable.
+            ramp = np.linspace(0.0, 1.0, len(Q))[:, None]
+            Qm = Q + (P[0] - Q[0]) * (1 - ramp) + (P[-1] - Q[-1]) * ramp
+            for PP, QQ in ((P, Q), (P, Qm)):
+                lb = O.endpoint_lb(PP, QQ)
+                dfd = O.dfd(PP, QQ)
+                say("dead skips", rows, d, lb)
+                with Intersection(PP, QQ, ctx=ctx) as s:
+                    s.data_move()
+                    ctx.reset_stats()
+                    for f in (0.0, 0.05, 0.2, 0.5, 0.8, 0.95, 1.0, 1.05):
+                        r = lb + f * (dfd - lb)
+                        assert s.decide_ub(r) == O.decide_ub(PP, QQ, r), (d, f)
+                    st = ctx.stats()["decide_f64"]
+                    assert st["evaluated"] < st["cells"]
+                    skipped += st["skipped_blocks"]
+        assert skipped > 0
     finally:
         ctx.close()

@@ -345,6 +345,28 @@ def test_sweep_rows_mt_matches_single_thread(kind, threads):
         assert mt == st, (r0, r1)
 
 
+@pytest.mark.parametrize("threads", [1, 2, 3, 8, 40])
+def test_sweep_mt_terminal_values_equal_single_thread(threads):
+    """The row-pipelined whole sweeps that produce the reference values of
+    long pairs are bit-identical to or_vefd, or_dfd and or_decide_ub (whose
+    decision computes every cell's intervals; the pipelined one skips cells
+    whose inputs are both unreachable): ragged row blocks, more threads than
+    rows, columns not a multiple of the 256-column chunk, 2D and 3D, radii from
+    the endpoint bound to past DFD and within 1e-9 of the distance."""
+    rng = np.random.default_rng(900 + threads)
+    shapes = [(2, 2, 2), (3, 2, 3), (2, 300, 2), (61, 700, 3), (300, 257, 2), (40, 513, 3)]
+    for m, n, d in shapes:
+        P, Q = walk(rng, m, d), walk(rng, n, d)
+        lb, dfd = O.endpoint_lb(P, Q), O.dfd(P, Q)
+        same = lambda a, b: a == b and math.copysign(1.0, a) == math.copysign(1.0, b)
+        assert same(O.sweep_mt(P, Q, 2, 0.0, threads), dfd), (m, n)
+        assert same(O.sweep_mt(P, Q, 0, lb, threads), O.vefd(P, Q, lb)), (m, n)
+        fd = O.distance(P, Q, stop_mode=1, eps=1e-12)["distance"]
+        radii = list(np.linspace(lb, dfd * 1.05 + 1e-12, 6)) + [fd * (1 - 1e-9), fd * (1 + 1e-9)]
+        for r in radii:
+            assert same(O.sweep_mt(P, Q, 1, r, threads), O.decide_ub(P, Q, r)), (m, n, r)
+
+
 # ---------------- two-phase decision form ----------------
 
 @pytest.mark.parametrize("seed", range(6))
@@ -412,3 +434,57 @@ def test_two_phase_distance_within_tolerance(seed):
     b = O.distance(P, Q, stop_mode=1, eps=1e-9, form=1)
     assert b["dfd"] == a["dfd"] and b["vefd"] == a["vefd"]
     assert b["distance"] == pytest.approx(a["distance"], rel=2e-9)
+
+
+# ---------------- refined upper bound (DFD of midpoint-refined curves) ----------------
+
+def numpy_refine(P, levels):
+    for _ in range(levels):
+        mid = P[:-1] + 0.5 * (P[1:] - P[:-1])
+        out = np.empty((2 * len(P) - 1, P.shape[1]))
+        out[0::2] = P
+        out[1::2] = mid
+        P = out
+    return P
+
+
+@pytest.mark.parametrize("levels", [0, 1, 2, 3])
+def test_refine_is_the_midpoint_rule(levels):
+    rng = np.random.default_rng(70 + levels)
+    for d in (2, 3):
+        P = walk(rng, 9, d)
+        got = O.refine(P, levels)
+        assert got.shape == ((9 - 1) * 2 ** levels + 1, d)
+        assert np.array_equal(got, numpy_refine(P, levels))
+    one = walk(rng, 1, 2)
+    assert np.array_equal(O.refine(one, levels), one)  # a single point has no segment to split
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_refined_dfd_is_eiter_mannila_of_refined_curves(seed):
+    rng = np.random.default_rng(80 + seed)
+    m, n = rng.integers(2, 9, 2)
+    P, Q = walk(rng, m, 2 + seed % 2), walk(rng, n, 2 + seed % 2)
+    for levels in (1, 2):
+        want = I.dfd_eiter_mannila(numpy_refine(P, levels), numpy_refine(Q, levels))
+        assert O.dfd_refined(P, Q, levels) == pytest.approx(want, rel=1e-15)
+    assert O.dfd_refined(P, Q, 0) == O.dfd(P, Q)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_refined_dfd_lies_between_fd_and_dfd(seed):
+    """FD <= DFD(refined) <= DFD, with FD the least Alt-Godau critical value
+    (endpoint, vertex-edge, monotonicity) that the independent decision
+    accepts; each level of refinement is at most the one before."""
+    rng = np.random.default_rng(90 + seed)
+    m, n = rng.integers(3, 7, 2)
+    P, Q = walk(rng, m, 2), walk(rng, n, 2)
+    lb = O.endpoint_lb(P, Q)
+    cand = [c for c in I.critical_values(P, Q) if c >= lb]
+    fd = min(c for c in cand if I.decide_altgodau(P, Q, c * (1 + 1e-11)))
+    prev = O.dfd(P, Q)
+    for levels in (1, 2, 3, 4):
+        ub = O.dfd_refined(P, Q, levels)
+        assert fd * (1 - 1e-12) <= ub <= prev * (1 + 1e-15)
+        prev = ub
+    assert prev - fd <= O.dfd(P, Q) - fd
