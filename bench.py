@@ -241,7 +241,7 @@ def refined_ub_section(pairs, open_mask, ctx, levels=(1, 2)) -> dict:
     return out
 
 
-def batch_section(count: int, ctx) -> dict:
+def batch_section(count: int, ctx, refined: bool = True) -> dict:
     """Pairs/s of the batched pipeline on `count` pairs (lengths log-uniform in
     [500, 20000] per curve, dims 2/3 at random, N(0,1) random walks, seed 0;
     the network never saw these pairs): plain sequential bisection,
@@ -306,7 +306,8 @@ def batch_section(count: int, ctx) -> dict:
             res["bound_round_ms"] = bound_ms
             res["bound_round_frac_of_plain"] = bound_ms / res["plain"]["ms"]
             res["speedup_ceiling_if_decisions_free"] = res["plain"]["ms"] / bound_ms
-        res["refined_ub"] = refined_ub_section(pairs, ref["iterations"] > 0, ctx)
+        if refined:
+            res["refined_ub"] = refined_ub_section(pairs, ref["iterations"] > 0, ctx)
         if net is not None:
             res["learned_policy"] = {"tau": net.policy[0], "delta": net.policy[1],
                                      "weights": os.path.relpath(DEFAULT_WEIGHTS, ROOT)}

@@ -210,9 +210,9 @@ __global__ __launch_bounds__(64) void batch_sweep(BatchArgs a) {
         job.watch = a.watch;
         // A decision item runs the interval form of its pair (wave-uniform).
         if (OP == OP_DEC && p.dec_form == DEC_TWO_PHASE)
-            run_band<OP, 3, double, SYNC_SCOPED, R, true>(job, band, qring, sring, evals, nullptr, skips);
+            run_band<OP, 3, double, SYNC_SCOPED, R, true, false>(job, band, qring, sring, evals, nullptr, skips);
         else
-            run_band<OP, 3, double, SYNC_SCOPED, R, PRUNE>(job, band, qring, sring, evals, nullptr, skips);
+            run_band<OP, 3, double, SYNC_SCOPED, R, PRUNE, false>(job, band, qring, sring, evals, nullptr, skips);
     }
     if (a.evals) atomicAdd(a.evals, evals);
 }

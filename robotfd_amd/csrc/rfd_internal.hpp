@@ -24,7 +24,7 @@ struct rfd_ctx {
     bool fused = true;            // FP64 bounds by the fused DFD + VE-FD sweep (RFD_FUSED=0 disables)
     int bounds_cfg = 0;           // fused sweep: 10 * rows per lane + producer waves, 0 = by size (RFD_BOUNDS)
     int wg_per_cu = 0;            // fused sweep: cap on workgroups per CU, 0 = none (RFD_WGCU)
-    unsigned spin_limit = 1u << 24;  // polls one hand-off wait may take before it gives up (RFD_SPIN_LIMIT)
+    unsigned spin_limit = 1u << 22;  // polls one hand-off wait may take before it gives up (RFD_SPIN_LIMIT)
     int batch_bounds = 1310;      // batches: fused bound round configuration, 0 = separate rounds (RFD_BATCH_BOUNDS)
     int lds_per_cu = 0;           // bytes of LDS per CU
     const char *tlog_path = nullptr;  // RFD_TLOG: append per-band timestamps of every sweep

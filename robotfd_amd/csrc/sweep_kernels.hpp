@@ -201,17 +201,23 @@ __device__ inline void iv_ends(T B, T disc, typename RootT<T>::type s, T inv2A, 
 }
 
 // Bounded hand-off waits.  Every wait of a sweep counts its polls; at `limit`
-// polls it raises the launch's abort word and gives up.  A wait also gives up
-// when it sees the word raised (read every 64th poll, so a wait that succeeds
-// within 63 polls never touches it), and a wave that gave up waits no more in
-// its band: it runs its remaining steps on whatever values are in its buffers
-// (every index stays within its ring or column), so every wave ends and the
-// kernel exits.  The host reads the word after the launch and reports the
-// sweep as failed.
+// polls it raises the launch's abort word and gives up.  A wave that gave up
+// waits no more in its band: it runs its remaining steps on whatever values
+// are in its buffers (every index stays within its ring or column), so every
+// wave ends and the kernel exits; the host reads the word after the launch and
+// reports the sweep as failed.  The first kQuickPolls polls of a wait touch no
+// memory and add only a count and a compare to the poll: an access to the word
+// would wait for the wave's own outstanding loads and stores, and heavier
+// polls take issue slots from the working waves that share a SIMD with many
+// waiting ones (a batch round's long chains of bands).  Past them a wait naps
+// longer between polls and gives up as soon as it sees the word raised (read
+// every 64th poll), so once one wait gives up, the others follow quickly.
 struct Watch {
     int *abort;       // 0 until a wait gives up
     unsigned limit;   // polls one wait may take (at least 1)
 };
+
+constexpr unsigned kQuickPolls = 1024;
 
 // One failed poll of a wait: true when the wait must give up.
 __device__ inline bool watch_expired(const Watch &w, unsigned &polls) {
@@ -219,9 +225,18 @@ __device__ inline bool watch_expired(const Watch &w, unsigned &polls) {
     ++polls;
     if (polls >= w.limit) {
         __hip_atomic_store(w.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // Nothing left in flight on this rare path: where it rejoins the
+        // common one, the compiler's counter waits stay those of the common
+        // path (an outstanding access here would make later waits stricter).
+        __builtin_amdgcn_s_waitcnt(0);
         return true;
     }
-    return (polls & 63) == 0 && __hip_atomic_load(w.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    if (polls < kQuickPolls) return false;
+    __builtin_amdgcn_s_sleep(16);  // a long wait: poll seldom
+    if ((polls & 63) != 0) return false;
+    const bool raised = __hip_atomic_load(w.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    __builtin_amdgcn_s_waitcnt(0);
+    return raised;
 }
 
 struct SweepArgs {
@@ -325,7 +340,11 @@ template <int OP> struct Staged { static constexpr int value = (OP == OP_DEC) ? 
 // VAR selects the operation's variant: VE-FD skips cells whose incoming value
 // is at least thr (exact pruning); the decision uses the two-phase interval
 // form (iv_disc / iv_ends with TWO_PHASE).
-template <int OP, int D, typename T, int SYNC, int R, bool VAR = false>
+// PIPE selects the decision step's form: true (single pair: a wave or two per
+// SIMD, latency-bound) computes every row's intervals without branches one
+// step ahead; false (batch rounds: several waves per SIMD, throughput-bound)
+// computes a row's intervals only where the row is live somewhere in the wave.
+template <int OP, int D, typename T, int SYNC, int R, bool VAR = false, bool PIPE = true>
 __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4][D][64],
                                 T (&sring)[4][Staged<OP>::value][64], unsigned long long &evals,
                                 unsigned long long *tlog, unsigned long long &skips) {
@@ -476,7 +495,9 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
             if (has_pred && s < cols) {
                 const long long need = (s + kHand<OP> < cols) ? s + kHand<OP> : cols;
                 unsigned polls = 0;
-                while (!gave_up && flag_load<SYNC>(&a.progress[band - 1]) < (int)need) {
+                // Every lane reads the same flag: taking lane 0's value makes
+                // the wait a scalar loop (no exec-mask bookkeeping per poll).
+                while (!gave_up && __builtin_amdgcn_readfirstlane(flag_load<SYNC>(&a.progress[band - 1])) < (int)need) {
                     if (watch_expired(a.watch, polls)) gave_up = true;
                     else __builtin_amdgcn_s_sleep(1);
                 }
@@ -548,6 +569,58 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
             // intervals were not computed the step before (the first live step
             // after dead ones) computes its own first.
             const bool inr = j >= 0 && j < cols && i0 < rows;
+          if constexpr (!PIPE) {
+            // Throughput form (batch rounds, several waves per SIMD): row by
+            // row, a row's free intervals computed only where one of its
+            // lanes is live (the branch skips rows dead across the wave).
+            if (inr) {
+                const int ra = (int)((j >> 6) & 3), oa = (int)(j & 63);
+                const long long jb = j + 1;
+                const int rb = (int)((jb >> 6) & 3), ob = (int)(jb & 63);
+                T qa[D], qb[D];
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    qa[k] = qring[ra][k][oa];
+                    qb[k] = qring[rb][k][ob];
+                }
+                const T A4b = sring[ra][0][oa];
+                const T inv2b = sring[ra][NS - 1][oa];
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    const long long it = i0 + t;
+                    if (R > 1 && it >= rows) break;
+                    const T leftLB = left, botLB = bot[t];
+                    T rLB = T(-1), rUB = T(-2), tLB = T(-1);
+                    // A NaN input (two-phase form) is not unreachable: the
+                    // test is on < 0, not on >= 0.
+                    if (!(leftLB < T(0) && botLB < T(0))) {
+                        ++evals;
+                        T B1, D1, B2, D2, tUB = T(-2);
+                        iv_disc<T, D, VAR>(p[t + 1], qa, qb, r2, A4b, B1, D1);
+                        iv_disc<T, D, VAR>(qb, p[t], p[t + 1], r2, A4a[t], B2, D2);
+                        // The root only where the segment meets the disc (a
+                        // branch: waves whose lanes all miss skip it).
+                        if (D1 >= T(0)) iv_ends<T, VAR>(B1, D1, iv_root(D1), inv2b, rLB, rUB);
+                        if (D2 >= T(0)) iv_ends<T, VAR>(B2, D2, iv_root(D2), inv2a[t], tLB, tUB);
+                        if (botLB < T(0)) {
+                            const bool inv = leftLB > rUB || leftLB < T(0);
+                            const T t2 = N::max(rLB, leftLB);
+                            rLB = inv ? T(-1) : t2;
+                            rUB = inv ? T(-2) : rUB;
+                        }
+                        if (leftLB < T(0)) {
+                            const bool inv = botLB > tUB || botLB < T(0);
+                            const T t2 = N::max(tLB, botLB);
+                            tLB = inv ? T(-1) : t2;
+                        }
+                    }
+                    bot[t] = tLB;
+                    left = rLB;
+                    if (it == rows - 1 && j == cols - 1) term = rUB;
+                }
+                out = left;
+            }
+          } else {
             bool maybe = inr && !(left < T(0));
 #pragma unroll
             for (int t = 0; t < R; ++t) maybe = maybe || (inr && !(bot[t] < T(0)));
@@ -602,6 +675,7 @@ __device__ inline void run_band(const SweepJob &a, const int band, T (&qring)[4]
                 out = inr ? T(-1) : out;
                 term = (inr && i0 <= rows - 1 && rows - 1 < i0 + R && j == cols - 1) ? T(-2) : term;
             }
+          }
             reload = false;
         }
         if (OP != OP_DEC && j >= 0 && j < cols && i0 < rows) {

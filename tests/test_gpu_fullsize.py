@@ -11,7 +11,7 @@ values the CPU oracle computed (tests/golden/make_fullsize_golden.py):
 * the 4,096-pair batch: every pair's DFD, VE-FD, distance and step count bit
   for bit under plain bisection; speculative bisection identical to plain; the
   learned bracket (committed weights), with and without speculation, within
-  1e-9 relative of the oracle's distance with lb <= ub for every pair.
+  1e-9 relative of the oracle's distance with lb <= ub wherever VE-FD <= DFD.
 """
 import json
 import os
@@ -111,9 +111,16 @@ def test_batch_4096_pairs_match_oracle(gpu):
                 lr = run_learned(b, net, rows_per_lane=1, spec_depth=depth)
                 rel = np.abs(lr["distance"] - z["distance"]) / z["distance"]
                 assert rel.max() <= 1e-9, (depth, int(rel.argmax()), rel.max())
-                assert (lr["lb"] <= lr["ub"]).all()
+                # VE-FD rounds one ulp above DFD on a few pairs (36 of 4,096:
+                # a point-segment weight one rounding above the vertex
+                # distance); their bracket is closed, as in the oracle.
+                inverted = z["vefd"] > z["dfd"]
+                assert (lr["lb"] <= lr["ub"])[~inverted].all()
+                assert (lr["lb"][inverted] == z["lb"][inverted]).all()
+                assert (lr["ub"][inverted] == z["ub"][inverted]).all()
                 # Both brackets hold the distance, so they overlap.
-                assert (lr["lb"] <= z["ub"]).all() and (lr["ub"] >= z["lb"]).all()
+                ok = ~inverted
+                assert (lr["lb"][ok] <= z["ub"][ok]).all() and (lr["ub"][ok] >= z["lb"][ok]).all()
                 assert (lr["dfd"] == z["dfd"]).all() and (lr["vefd"] == z["vefd"]).all()
         finally:
             net.close()
