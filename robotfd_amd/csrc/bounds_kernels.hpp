@@ -897,12 +897,16 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         // per lane, as a ring of 4 does with pairs) and 16% longer (4 rows).
         static_assert(KS >= 3, "step pairs need a ring of three steps");
         constexpr bool QUADS = KS >= 12 && NPW >= 5;
-        auto one = [&](const int s, const double(&qa)[D], double(&qb)[D], const double(&xp)[RP][D],
+        // The boundary rules apply in the first 64 steps (lanes before or at
+        // their first column) and in the first and last bands (row 0, padding
+        // rows): those steps run a copy of the loop that checks them, the
+        // others one without the check.
+        auto one = [&](auto edge_c, const int s, const double(&qa)[D], double(&qb)[D], const double(&xp)[RP][D],
                        double(&xn)[RP][D], const bool first, const bool last, const int ahead)
                        __attribute__((always_inline)) {
             if (first) step_begin(s, last ? 0 : ahead);
             double *w = wring + slot * (R * 3 * 64) + lane;
-            column(s - lane, qa, qb, xp, xn, w, s < 64 || edge_band);
+            column(s - lane, qa, qb, xp, xn, w, decltype(edge_c)::value);
             if (last) {
                 step_end(s);
             } else {
@@ -910,19 +914,28 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
             }
         };
         int s = 0;
-        if constexpr (QUADS) {
-            for (; s + 3 < nsteps; s += 4) {
-                one(s, qA, qB, xA, xB, true, false, 3);
-                one(s + 1, qB, qA, xB, xA, false, false, 0);
-                one(s + 2, qA, qB, xA, xB, false, false, 0);
-                one(s + 3, qB, qA, xB, xA, false, true, 0);
+        auto steps_to = [&](auto e, const int end) __attribute__((always_inline)) {
+            if constexpr (QUADS) {
+                for (; s + 3 < end; s += 4) {
+                    one(e, s, qA, qB, xA, xB, true, false, 3);
+                    one(e, s + 1, qB, qA, xB, xA, false, false, 0);
+                    one(e, s + 2, qA, qB, xA, xB, false, false, 0);
+                    one(e, s + 3, qB, qA, xB, xA, false, true, 0);
+                }
             }
-        }
-        for (; s + 1 < nsteps; s += 2) {
-            one(s, qA, qB, xA, xB, true, false, 1);
-            one(s + 1, qB, qA, xB, xA, false, true, 0);
-        }
-        if (s < nsteps) one(s, qA, qB, xA, xB, true, true, 0);
+            for (; s + 1 < end; s += 2) {
+                one(e, s, qA, qB, xA, xB, true, false, 1);
+                one(e, s + 1, qB, qA, xB, xA, false, true, 0);
+            }
+            if (s < end) {
+                one(e, s, qA, qB, xA, xB, true, true, 0);
+                ++s;
+            }
+        };
+        // 64 is a multiple of the trip length, so the second loop starts a
+        // trip at a chunk boundary, as the first does.
+        steps_to(std::true_type{}, edge_band ? nsteps : (nsteps < 64 ? nsteps : 64));
+        steps_to(std::false_type{}, nsteps);
     }
 }
 
