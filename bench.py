@@ -331,6 +331,21 @@ def profile_mode(what: str, points: int) -> int:
         print(json.dumps({"profile": "copy", "bytes_read": nbytes, "bytes_written": nbytes, "copies": 3}))
         return 0
     from robotfd_amd import Context, Intersection, workloads as W
+    if what == "learned":
+        # One learned-bracket run of the 4,096-pair batch (its network layers
+        # are the MFMA kernels).
+        from robotfd_amd.batch import Batch
+        from robotfd_amd.learned import DEFAULT_WEIGHTS, BracketNet, run_learned
+        ctx = Context.default()
+        with Batch(W.batch_pairs(4096, seed=0), ctx=ctx) as b:
+            net = BracketNet(ctx=ctx, path=DEFAULT_WEIGHTS)
+            try:
+                r = run_learned(b, net, spec_depth=0)
+            finally:
+                net.close()
+        print(json.dumps({"profile": what, "pairs": 4096, "iterations": int(r["iterations"].sum()),
+                          "kernels": ctx.stats()}))
+        return 0
     P, Q = W.long_3d(points, points, seed=0)
     ctx = Context.default()
     with Intersection(P, Q, ctx=ctx) as s:
@@ -358,10 +373,11 @@ def main() -> int:
                     help="skip the mixed-precision run reported beside the FP64 line")
     ap.add_argument("--batch-pairs", type=int, default=4096,
                     help="pairs of the batched workload reported beside the main line (0: skip)")
-    ap.add_argument("--profile", choices=["decision", "bounds", "copy"],
+    ap.add_argument("--profile", choices=["decision", "bounds", "copy", "learned"],
                     help="run one launch of a kernel for rocprofv3 and exit: the FP64 decision at "
                          "DFD (1 - 0.5e-9) or the fused bound sweep on the pair of --points, or a "
-                         "256 MiB device-to-device copy (byte-counter calibration)")
+                         "256 MiB device-to-device copy (byte-counter calibration), or one "
+                         "learned-bracket run of the 4,096-pair batch")
     args = ap.parse_args()
     if args.profile:
         return profile_mode(args.profile, args.points)

@@ -111,9 +111,11 @@ def test_batch_4096_pairs_match_oracle(gpu):
                 lr = run_learned(b, net, rows_per_lane=1, spec_depth=depth)
                 rel = np.abs(lr["distance"] - z["distance"]) / z["distance"]
                 assert rel.max() <= 1e-9, (depth, int(rel.argmax()), rel.max())
-                # VE-FD rounds one ulp above DFD on a few pairs (36 of 4,096:
-                # a point-segment weight one rounding above the vertex
-                # distance); their bracket is closed, as in the oracle.
+                # VE-FD rounds one ulp above DFD on 36 of the 4,096 pairs:
+                # their DFD is attained at an end pair, and the endpoint bound
+                # VE-FD starts from (Point::dist_sqr's operation order) rounds
+                # an ulp above DFD's FMA chain.  Their bracket is closed, as in
+                # the oracle.
                 inverted = z["vefd"] > z["dfd"]
                 assert (lr["lb"] <= lr["ub"])[~inverted].all()
                 assert (lr["lb"][inverted] == z["lb"][inverted]).all()
