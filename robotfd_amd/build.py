@@ -20,7 +20,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "librfd.so")
 
-HIP_SOURCES = ["rfd_capi.hip", "rfd_batch.hip", "rfd_bounds.hip", "rfd_mlp.hip"]
+HIP_SOURCES = ["rfd_capi.hip", "rfd_batch.hip", "rfd_bounds.hip", "rfd_dec.hip", "rfd_mlp.hip"]
 # Per-file additions: the fused bound sweep's unit has no NaN on its path and
 # is compiled without NaN semantics (see the head of rfd_bounds.hip).
 EXTRA_FLAGS = {"rfd_bounds.hip": ["-fno-honor-nans"]}

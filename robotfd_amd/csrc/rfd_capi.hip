@@ -9,6 +9,7 @@
 #include "rfd_internal.hpp"
 #include "sweep_kernels.hpp"
 #include "bounds_kernels.hpp"
+#include "dec_types.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -407,6 +408,98 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
     return RFD_OK;
 }
 
+// Flags of the windowed decision sweep a session's progress array holds (one
+// 64-bit word per band).
+int64_t dec_flag_capacity(const rfd_session *s) {
+    return s->noalloc ? kNaProgressBytes / 8 : nbands_for(s->m) + 2;
+}
+
+// One windowed decision sweep (FP64) of a session (m, n >= 2): the terminal
+// rightUB at `radius`.  false in *ran when the configuration does not fit the
+// session's flag array (the caller then runs band_sweep).
+rfd_status run_dec(rfd_session *s, double radius, double *result, bool *ran) {
+    rfd_ctx *ctx = s->ctx;
+    *ran = false;
+    const int64_t rows = s->m - 1, cols = s->n - 1;
+    // By size: six producer waves and the rows per lane that give about three
+    // bands per CU.  Measured near the critical radius: 100k x 120k fastest at
+    // 2 rows per lane (26.8 ms; 36 ms at 4), 200k x 200k at 4 (67.6 ms; 72.6
+    // at 2); one workgroup per CU (two: 75 ms at 200k).
+    int cfg = ctx->dec_cfg;
+    if (cfg < 0) {
+        const int64_t per = 192 * (int64_t)std::max(ctx->cu_count, 1);  // rows of three 1-row bands per CU
+        const int64_t r = (rows + per / 2) / per;
+        cfg = 10 * (int)std::max<int64_t>(1, std::min<int64_t>(4, r)) + 6;
+    }
+    const int R = cfg / 10;
+    const int64_t nb = (rows + 64 * R - 1) / (64 * R);
+    if (nb > dec_flag_capacity(s) || cols >= ((int64_t)1 << 30)) return RFD_OK;
+    *ran = true;
+    rfd::DecArgs a{};
+    a.P = s->dP;
+    a.Q = s->dQ;
+    a.qseg = s->qseg64;
+    a.slots = s->slots;
+    a.flags = reinterpret_cast<unsigned long long *>(s->progress);
+    a.queue = s->queue;
+    a.result = s->result;
+    a.rows = rows;
+    a.cols = cols;
+    a.nbands = (int)nb;
+    a.r = radius;
+    a.evals = s->evals;
+    a.watch = rfd::Watch{s->abort, ctx->spin_limit};
+    const double sentinel = -3.0;  // .cu:193 (result = -3 until the terminal cell writes it)
+    unsigned long long *tlog = nullptr;
+    if (ctx->tlog_path && !s->noalloc) {
+        RFD_HIP(ctx, hipMalloc(&tlog, sizeof(unsigned long long) * 12 * nb));
+        RFD_HIP(ctx, hipMemsetAsync(tlog, 0, sizeof(unsigned long long) * 12 * nb, ctx->stream));
+    }
+    a.tlog = tlog;
+    RFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    rfd_host::rfd_launch_dec_init(ctx, a, sentinel, s->abort);
+    RFD_HIP(ctx, hipGetLastError());
+    int grid = 0;
+    RFD_HIP(ctx, rfd_host::rfd_launch_dec(ctx, s->dims, cfg, s->dec_form == rfd::DEC_TWO_PHASE, nb, a, &grid));
+    RFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    double h = 0;
+    unsigned long long hev[2] = {0, 0};
+    int habort = 0;
+    RFD_HIP(ctx, hipMemcpyAsync(&h, s->result, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RFD_HIP(ctx, hipMemcpyAsync(hev, s->evals, sizeof(hev), hipMemcpyDeviceToHost, ctx->stream));
+    RFD_HIP(ctx, hipMemcpyAsync(&habort, s->abort, sizeof(habort), hipMemcpyDeviceToHost, ctx->stream));
+    RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    RFD_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.launches[RFD_K_DEC64] += 1;
+    ctx->stats.ms[RFD_K_DEC64] += ms;
+    ctx->stats.cells[RFD_K_DEC64] += (double)rows * (double)cols;
+    ctx->stats.evaluated[RFD_K_DEC64] += (double)hev[0];
+    ctx->stats.skipped_blocks[RFD_K_DEC64] += (double)hev[1];
+    if (tlog) {
+        std::vector<unsigned long long> h_t((size_t)(12 * nb));
+        RFD_HIP(ctx, hipMemcpy(h_t.data(), tlog, sizeof(unsigned long long) * 12 * nb, hipMemcpyDeviceToHost));
+        RFD_HIP(ctx, hipFree(tlog));
+        if (FILE *f = std::fopen(ctx->tlog_path, "ab")) {
+            long long hdr[6] = {(long long)RFD_K_DEC64, (long long)rows, (long long)cols, (long long)nb,
+                                (long long)grid, (long long)(100000 + cfg)};
+            std::fwrite(hdr, sizeof(hdr), 1, f);
+            std::fwrite(h_t.data(), sizeof(unsigned long long), h_t.size(), f);
+            std::fclose(f);
+        }
+    }
+    if (habort) {
+        set_err(ctx, "windowed decision sweep: a hand-off wait exceeded %u polls and gave up", ctx->spin_limit);
+        return RFD_ERR_STATE;
+    }
+    if (h == sentinel) {
+        set_err(ctx, "windowed decision sweep wrote no terminal value");
+        return RFD_ERR_STATE;
+    }
+    *result = h;
+    return RFD_OK;
+}
+
 // Midpoint refinement of a device curve, `levels` times, into a new device
 // array (*out, *len_out); the caller frees it.
 rfd_status refine_curve(rfd_ctx *ctx, const double *src, int64_t len, int dims, int levels, double **out,
@@ -590,6 +683,11 @@ rfd_status rfd_create(rfd_ctx **out, int device) {
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) == hipSuccess)
             ctx->lds_per_cu = lds;
     }
+    if (const char *e = std::getenv("RFD_DEC")) {
+        const int c = std::atoi(e);
+        if (c == 0 || rfd_host::rfd_valid_dec_cfg(c)) ctx->dec_cfg = c;
+    }
+    if (const char *e = std::getenv("RFD_DEC_WGCU")) ctx->dec_wg_per_cu = std::max(0, std::atoi(e));
     if (const char *e = std::getenv("RFD_ROWS")) {
         int r = std::atoi(e);
         if (valid_rows_per_lane(r)) ctx->rows_per_lane = r;
@@ -670,9 +768,10 @@ rfd_status rfd_session_create(rfd_ctx *ctx, const double *P, int64_t m, const do
     if ((e = hipMalloc(&s->dP, sizeof(double) * m * dims)) != hipSuccess) return fail(e, "hipMalloc(P)");
     if ((e = hipMalloc(&s->dQ, sizeof(double) * n * dims)) != hipSuccess) return fail(e, "hipMalloc(Q)");
     // Band-boundary buffers: two columns of n doubles (two kinds each for the
-    // fused bound sweep); progress flags: one per band.
+    // fused bound sweep); progress flags: one per band of one row per lane
     if ((e = hipMalloc(&s->slots, sizeof(double) * 4 * n)) != hipSuccess) return fail(e, "hipMalloc(slots)");
-    if ((e = hipMalloc(&s->progress, sizeof(int) * (nbands_for(m) + 1))) != hipSuccess)
+    // (the windowed decision sweep keeps a 64-bit word per band there).
+    if ((e = hipMalloc(&s->progress, sizeof(int) * 2 * (nbands_for(m) + 2))) != hipSuccess)
         return fail(e, "hipMalloc(progress)");
     if ((e = hipMalloc(&s->queue, sizeof(int))) != hipSuccess) return fail(e, "hipMalloc(queue)");
     if ((e = hipMalloc(&s->result, 4 * sizeof(double))) != hipSuccess) return fail(e, "hipMalloc(result)");
@@ -747,7 +846,10 @@ rfd_status rfd_session_decide(rfd_session *s, double radius, int fp32, int *reac
         st = run_sweep<rfd::OP_DEC, float>(s, s->fP, s->fQ, s->qseg32, radius, RFD_K_DEC32, &ub);
     } else {
         if ((st = ensure_qseg<double>(s, &s->qseg64, s->dQ)) != RFD_OK) return st;
-        st = run_sweep<rfd::OP_DEC, double>(s, s->dP, s->dQ, s->qseg64, radius, RFD_K_DEC64, &ub);
+        bool ran = false;
+        if (ctx->dec_cfg != 0 && ctx->sync == rfd::SYNC_SCOPED && (st = run_dec(s, radius, &ub, &ran)) != RFD_OK)
+            return st;
+        if (!ran) st = run_sweep<rfd::OP_DEC, double>(s, s->dP, s->dQ, s->qseg64, radius, RFD_K_DEC64, &ub);
     }
     if (st != RFD_OK) return st;
     *reachable = (ub == 1.0) ? 1 : 0;  // .cu:1592

@@ -26,6 +26,9 @@ struct rfd_ctx {
     int wg_per_cu = 0;            // fused sweep: cap on workgroups per CU, 0 = none (RFD_WGCU)
     unsigned spin_limit = 1u << 22;  // polls one hand-off wait may take before it gives up (RFD_SPIN_LIMIT)
     int batch_bounds = 1310;      // batches: fused bound round configuration, 0 = separate rounds (RFD_BATCH_BOUNDS)
+    int dec_cfg = -1;             // FP64 decisions: windowed sweep, 10 * rows per lane + producer waves; -1 = by size,
+                                  // 0 = band_sweep (RFD_DEC)
+    int dec_wg_per_cu = 1;        // windowed decision sweep: workgroups per CU, 0 = as many as fit (RFD_DEC_WGCU)
     int lds_per_cu = 0;           // bytes of LDS per CU
     const char *tlog_path = nullptr;  // RFD_TLOG: append per-band timestamps of every sweep
     void *scratch = nullptr;      // device address of the library's fixed working array on this device
@@ -48,6 +51,7 @@ inline void set_err(rfd_ctx *ctx, const char *fmt, ...) {
 namespace rfd {
 struct BoundsArgs;
 struct BatchArgs;
+struct DecArgs;
 }  // namespace rfd
 
 namespace rfd_host {
@@ -59,6 +63,13 @@ bool rfd_valid_bounds_cfg(int cfg);
 hipError_t rfd_launch_bounds(rfd_ctx *ctx, int dims, int cfg, int64_t nb, const rfd::BoundsArgs &a, int *grid);
 bool rfd_valid_batch_bounds_cfg(int cfg);
 void rfd_launch_batch_bounds(rfd_ctx *ctx, int cfg, hipStream_t st, const rfd::BatchArgs &a);
+
+// The windowed decision sweep's launches (rfd_dec.hip).  Configuration codes:
+// 10 * rows per lane + producer waves.
+bool rfd_valid_dec_cfg(int cfg);
+hipError_t rfd_launch_dec(rfd_ctx *ctx, int dims, int cfg, bool two_phase, int64_t nb, const rfd::DecArgs &a,
+                          int *grid);
+void rfd_launch_dec_init(rfd_ctx *ctx, const rfd::DecArgs &a, double sentinel, int *abort);
 
 // The reference's choice of decision path for an m x n pair from free device
 // memory (rfd_capi.hip): rfd::DEC_TWO_PHASE or rfd::DEC_FUSED.

@@ -167,8 +167,9 @@ DEC_SHAPES = [(2, 2), (2, 3), (3, 2), (5, 7), (65, 66), (130, 200), (300, 257), 
 @pytest.mark.parametrize("dims", [2, 3])
 def test_decision_rows_per_lane_bit_exact(contexts, rows, dims):
     """The decision sweep's terminal rightUB against the oracle's at every
-    rows-per-lane choice, at radii from the endpoint bound to past DFD."""
-    ctx = contexts(RFD_ROWS=rows)
+    rows-per-lane choice, at radii from the endpoint bound to past DFD
+    (band_sweep's decision, RFD_DEC=0)."""
+    ctx = contexts(RFD_ROWS=rows, RFD_DEC=0)
     ctx.reset_stats()
     calls = 0
     for m, n in DEC_SHAPES:
@@ -186,8 +187,8 @@ def test_decision_rows_per_lane_bit_exact(contexts, rows, dims):
 @pytest.mark.parametrize("rows", [1, 3])
 def test_decision_near_critical_value_and_wrap(contexts, rows):
     """Waves taking several bands each (3 waves), at radii within 1e-9 and
-    1e-6 of the distance."""
-    ctx = contexts(RFD_ROWS=rows, RFD_MAX_WAVES=3)
+    1e-6 of the distance (band_sweep's decision, RFD_DEC=0)."""
+    ctx = contexts(RFD_ROWS=rows, RFD_MAX_WAVES=3, RFD_DEC=0)
     P, Q = W.robot_pair(64 * 13 * 4 + 7, 700, seed=2)
     res = O.distance(P, Q, stop_mode=1, eps=1e-12)
     fd = res["distance"]

@@ -179,11 +179,14 @@ def test_decision_dead_skips_across_bands(rows):
     blocks can be dead, skipped, and handed to the next band as dead columns.
     The terminal rightUB stays bit-exact; dead cells do occur (fewer cells
     evaluated than swept) and whole dead blocks are skipped (counted)."""
+    # band_sweep's decision (RFD_DEC=0; the windowed sweep has its own tests).
     os.environ["RFD_ROWS"] = str(rows)
+    os.environ["RFD_DEC"] = "0"
     try:
         ctx = Context()
     finally:
         del os.environ["RFD_ROWS"]
+        del os.environ["RFD_DEC"]
     try:
         rng = np.random.default_rng(40 + rows)
         skipped = 0
