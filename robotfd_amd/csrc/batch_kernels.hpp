@@ -14,6 +14,7 @@
 #pragma once
 
 #include "batch_types.hpp"
+#include "dec_types.hpp"
 #include "sweep_kernels.hpp"
 
 namespace rfd {
@@ -77,7 +78,10 @@ __global__ __launch_bounds__(1024) void batch_plan(BatchArgs a, int phase) {
     const int tid = (int)threadIdx.x;
     const int per = (a.npairs + 1023) / 1024;
     const int k0 = tid * per;
-    const int R = (phase == PH_BOUNDS) ? a.bounds_rows : a.rows_per_lane;
+    // Decision rounds on the windowed sweep keep 64-bit band words instead
+    // of the progress flags.
+    const bool win = (phase == PH_DEC || phase == PH_CERT) && a.dec_win > 0;
+    const int R = (phase == PH_BOUNDS) ? a.bounds_rows : win ? a.dec_win / 10 : a.rows_per_lane;
     long long mine = 0;
     for (int q = 0; q < per; ++q) {
         const int idx = k0 + q;
@@ -154,7 +158,8 @@ __global__ __launch_bounds__(1024) void batch_plan(BatchArgs a, int phase) {
             if (!(ok & (1u << node))) continue;
             for (int b = 0; b < nb; ++b) {
                 a.items[pos + b] = make_int2(k * kSpec + node, b);
-                a.progress[p.bandoff + node * p.bandstride + b] = 0;
+                if (win) a.flags64[p.bandoff + node * p.bandstride + b] = flag64(0u, kLoUnknown);
+                else a.progress[p.bandoff + node * p.bandstride + b] = 0;
             }
             pos += nb;
             st.result[node][0] = -3.0;

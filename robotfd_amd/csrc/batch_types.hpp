@@ -65,6 +65,7 @@ struct BatchArgs {
     const double *qseg;
     double *slots;
     int *progress;
+    unsigned long long *flags64;  // windowed decision rounds: band words, indexed as progress
     int2 *items;             // (pair, band) work list of the current round
     const int2 *bitems;      // the fused bound round's work list, fixed per batch (or null)
     int *count;              // [0] items, [1] queue, [2] speculation depth of the round
@@ -77,6 +78,7 @@ struct BatchArgs {
     int prune;
     int gate;                // 1: VE-FD only when m*n > 10000
     int max_waves;           // waves of a sweep launch (sets the speculation depth)
+    int dec_win;             // decision rounds on the windowed sweep: 10 * rows per lane + producer waves; 0: band_sweep
     int max_depth;           // 1 = plain sequential bisection
     Watch watch;             // bound on the hand-off waits (abort word cleared by batch_reset)
 };

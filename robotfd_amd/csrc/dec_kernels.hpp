@@ -678,37 +678,51 @@ __device__ inline void dec_producer(const DecArgs &a, const int band, const int 
     }
 }
 
+// One band by every wave of the workgroup: wave 0 the chain, the courier in
+// the fifth slot (the chain's SIMD: waves are dealt to the CU's SIMDs in turn,
+// and the courier mostly sleeps) when there is one, the others producers.
+// The caller has cleared the hand-off words and ends with a barrier.
+template <int D, typename T, int R, int NP, bool VAR>
+__device__ inline void dec_roles(const DecArgs &a, const int band, DecShared<D, T, R, NP> &sh,
+                                 unsigned long long &evals, unsigned long long &skips) {
+    const int wave = (int)(threadIdx.x >> 6);
+    constexpr int cw = (NP + 2 > 4) ? 4 : NP + 1;
+    if (wave == 0) {
+        dec_chain<D, T, R, NP, VAR>(a, band, sh, evals, skips);
+    } else if (wave == cw) {
+        dec_courier<D, T, R, NP, VAR>(a, band, sh);
+    } else {
+        const int w = wave - 1 - (wave > cw ? 1 : 0);
+        dec_producer<D, T, R, NP, VAR>(a, band, w, sh);
+    }
+}
+
+template <int D, typename T, int R, int NP>
+__device__ inline void dec_clear(DecShared<D, T, R, NP> &sh) {
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 16; ++k) sh.prod[k] = 0;
+        sh.start = -2;
+        sh.stop = kDecNone;
+        sh.in_avail = 0;
+        sh.in_hi = kDecNone;
+        sh.out_done = 0;
+    }
+    if (threadIdx.x < 64) sh.done[threadIdx.x] = 0;
+}
+
 template <int D, typename T, int R, int NP, bool VAR>
 __global__ __launch_bounds__(64 * (NP + 2)) void dec_sweep(DecArgs a) {
     __shared__ DecShared<D, T, R, NP> sh;
     const int wave = (int)(threadIdx.x >> 6);
     const int lane = (int)(threadIdx.x & 63);
     unsigned long long evals = 0, skips = 0;
-    // Waves are dealt to the CU's SIMDs in turn: the courier, which mostly
-    // sleeps, takes the fifth slot (the chain's SIMD) when there is one.
-    constexpr int cw = (NP + 2 > 4) ? 4 : NP + 1;
     for (;;) {
-        if (threadIdx.x == 0) {
-            sh.item = atomicAdd(a.queue, 1);
-            for (int k = 0; k < 16; ++k) sh.prod[k] = 0;
-            sh.start = -2;
-            sh.stop = kDecNone;
-            sh.in_avail = 0;
-            sh.in_hi = kDecNone;
-            sh.out_done = 0;
-        }
-        if (threadIdx.x < 64) sh.done[threadIdx.x] = 0;
+        if (threadIdx.x == 0) sh.item = atomicAdd(a.queue, 1);
+        dec_clear(sh);
         __syncthreads();
         const int band = sh.item;
         if (band >= a.nbands) break;
-        if (wave == 0) {
-            dec_chain<D, T, R, NP, VAR>(a, band, sh, evals, skips);
-        } else if (wave == cw) {
-            dec_courier<D, T, R, NP, VAR>(a, band, sh);
-        } else {
-            const int w = wave - 1 - (wave > cw ? 1 : 0);
-            dec_producer<D, T, R, NP, VAR>(a, band, w, sh);
-        }
+        dec_roles<D, T, R, NP, VAR>(a, band, sh, evals, skips);
         if (a.tlog && threadIdx.x == 0) {
             a.tlog[12 * band + 2] = wall_clock64();
             a.tlog[12 * band + 3] = ((unsigned long long)__builtin_amdgcn_s_getreg((20 << 0) | (0 << 6) | (31 << 11)) << 32) |
@@ -719,21 +733,6 @@ __global__ __launch_bounds__(64 * (NP + 2)) void dec_sweep(DecArgs a) {
     if (wave == 0 && a.evals) {
         atomicAdd(a.evals, evals);
         if (lane == 0) atomicAdd(a.evals + 1, skips);
-    }
-}
-
-// Flags, queue, result and counters of one windowed decision sweep.
-__global__ void dec_init(unsigned long long *flags, int nbands, int *queue, double *result, double sentinel,
-                         unsigned long long *evals, int *abort) {
-    const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int stride = (int)(gridDim.x * blockDim.x);
-    for (int k = tid; k < nbands; k += stride) flags[k] = flag64(0u, kLoUnknown);
-    if (tid == 0) {
-        *queue = 0;
-        result[0] = sentinel;
-        evals[0] = 0;
-        evals[1] = 0;
-        *abort = 0;
     }
 }
 

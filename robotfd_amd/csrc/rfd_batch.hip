@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "batch_dec.hpp"
 #include "batch_kernels.hpp"
 #include "mlp_internal.hpp"
 #include "rfd.h"
@@ -38,10 +39,11 @@ struct GraphKey {
     const void *net;   // learned runs: the network whose forward pass is captured
     double tau, delta;
     int bcfg;          // fused bound round configuration (0: separate DFD and VE-FD rounds)
+    int dwin;          // decision rounds on the windowed sweep (configuration; 0: band_sweep)
     bool operator<(const GraphKey &o) const {
-        return std::tie(rpl, stop_mode, prune, gate, rounds, with_bounds, depth, eps, net, tau, delta, bcfg) <
+        return std::tie(rpl, stop_mode, prune, gate, rounds, with_bounds, depth, eps, net, tau, delta, bcfg, dwin) <
                std::tie(o.rpl, o.stop_mode, o.prune, o.gate, o.rounds, o.with_bounds, o.depth, o.eps,
-                        o.net, o.tau, o.delta, o.bcfg);
+                        o.net, o.tau, o.delta, o.bcfg, o.dwin);
     }
 };
 
@@ -61,6 +63,7 @@ struct rfd_batch {
     int *d_order = nullptr;
     double *pts = nullptr, *qseg = nullptr, *slots = nullptr;
     int *progress = nullptr;
+    unsigned long long *flags64 = nullptr;  // band words of windowed decision rounds
     int2 *items = nullptr;
     int2 *bitems = nullptr;    // the fused bound round's fixed work list
     int *count = nullptr;
@@ -99,6 +102,40 @@ void launch_batch_sweep(int rpl, int grid, hipStream_t st, const rfd::BatchArgs 
     else launch_batch_sweep_p<OP, false>(rpl, grid, st, a);
 }
 
+template <int R, int NP>
+int batch_dec_grid_cfg(rfd_ctx *ctx) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rfd::batch_dec<R, NP>, 64 * (NP + 2), 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    if (ctx->batch_dec_wgcu > 0) per_cu = std::min(per_cu, ctx->batch_dec_wgcu);
+    return std::max(1, std::min(ctx->cu_count * per_cu, ctx->max_waves));
+}
+
+// Workgroups of a windowed batch decision round.
+int rfd_batch_dec_grid(rfd_ctx *ctx) {
+    switch (ctx->batch_dec) {
+        case 13: return batch_dec_grid_cfg<1, 3>(ctx);
+        case 16: return batch_dec_grid_cfg<1, 6>(ctx);
+        default: return batch_dec_grid_cfg<2, 6>(ctx);
+    }
+}
+
+void launch_batch_dec(rfd_ctx *ctx, hipStream_t st, const rfd::BatchArgs &a) {
+    const int grid = rfd_batch_dec_grid(ctx);
+    switch (a.dec_win) {
+        case 13: hipLaunchKernelGGL((rfd::batch_dec<1, 3>), dim3(grid), dim3(64 * 5), 0, st, a); break;
+        case 16: hipLaunchKernelGGL((rfd::batch_dec<1, 6>), dim3(grid), dim3(64 * 8), 0, st, a); break;
+        default: hipLaunchKernelGGL((rfd::batch_dec<2, 6>), dim3(grid), dim3(64 * 8), 0, st, a); break;
+    }
+}
+
+// A decision or certification round's sweep.
+void launch_decisions(rfd_ctx *ctx, int rpl, hipStream_t st, const rfd::BatchArgs &a) {
+    if (a.dec_win) launch_batch_dec(ctx, st, a);
+    else launch_batch_sweep<rfd::OP_DEC>(rpl, ctx->max_waves, st, a);
+}
+
 rfd::BatchArgs make_args(rfd_batch *b, int rpl, const rfd_opts &o, int prune) {
     rfd::BatchArgs a{};
     a.bounds_rows = b->ctx->batch_bounds >= 1000 ? b->ctx->batch_bounds / 100 : b->ctx->batch_bounds / 10;
@@ -110,6 +147,7 @@ rfd::BatchArgs make_args(rfd_batch *b, int rpl, const rfd_opts &o, int prune) {
     a.qseg = b->qseg;
     a.slots = b->slots;
     a.progress = b->progress;
+    a.flags64 = b->flags64;
     a.items = b->items;
     a.bitems = b->bitems;
     a.count = b->count;
@@ -121,6 +159,10 @@ rfd::BatchArgs make_args(rfd_batch *b, int rpl, const rfd_opts &o, int prune) {
     a.prune = prune;
     a.gate = o.vefd_gate;
     a.max_waves = b->ctx->max_waves;
+    a.dec_win = b->ctx->batch_dec;
+    // Speculation depth: windowed rounds count workgroups (several waves
+    // each, one band at a time) instead of single-wave bands.
+    if (a.dec_win) a.max_waves = rfd_batch_dec_grid(b->ctx) * b->ctx->batch_dec_budget;
     a.max_depth = o.spec_depth >= 1 ? std::min(o.spec_depth, rfd::kMaxDepth) : rfd::kMaxDepth;
     a.watch = rfd::Watch{b->abort, b->ctx->spin_limit};
     return a;
@@ -163,13 +205,13 @@ void enqueue_rounds(rfd_batch *b, const rfd::BatchArgs &a, bool with_bounds, int
             hipLaunchKernelGGL(rfd::batch_learned_prepare, dim3(pg), dim3(256), 0, st, a,
                                (const float *)b->pred, tau, delta);
             hipLaunchKernelGGL(rfd::batch_plan, dim3(1), dim3(1024), 0, st, a, (int)rfd::PH_CERT);
-            launch_batch_sweep<rfd::OP_DEC>(rpl, grid, st, a);
+            launch_decisions(ctx, rpl, st, a);
             hipLaunchKernelGGL(rfd::batch_update, dim3(pg), dim3(256), 0, st, a, (int)rfd::PH_CERT);
         }
     }
     for (int r = 0; r < rounds; ++r) {
         hipLaunchKernelGGL(rfd::batch_plan, dim3(1), dim3(1024), 0, st, a, (int)rfd::PH_DEC);
-        launch_batch_sweep<rfd::OP_DEC>(rpl, grid, st, a);
+        launch_decisions(ctx, rpl, st, a);
         hipLaunchKernelGGL(rfd::batch_update, dim3(pg), dim3(256), 0, st, a, (int)rfd::PH_DEC);
     }
 }
@@ -208,7 +250,7 @@ rfd_status run_once(rfd_batch *b, const rfd_opts &o, int rpl, int prune,
     if (o.use_graph) {
         hipGraphExec_t ge;
         GraphKey key{rpl, o.stop_mode, prune, o.vefd_gate, kDecisionRounds, 1, a.max_depth, o.eps,
-                     net, tau, delta, ctx->batch_bounds};
+                     net, tau, delta, ctx->batch_bounds, ctx->batch_dec};
         rfd_status s = get_graph(b, key, a, &ge, net);
         if (s != RFD_OK) return s;
         RFD_HIP(ctx, hipGraphLaunch(ge, ctx->stream));
@@ -237,7 +279,7 @@ rfd_status run_once(rfd_batch *b, const rfd_opts &o, int rpl, int prune,
         if (o.use_graph) {
             hipGraphExec_t ge;
             GraphKey key{rpl, o.stop_mode, prune, o.vefd_gate, kContinueRounds, 0, a.max_depth, o.eps,
-                         nullptr, 0.0, 0.0, ctx->batch_bounds};
+                         nullptr, 0.0, 0.0, ctx->batch_bounds, ctx->batch_dec};
             rfd_status s = get_graph(b, key, a, &ge);
             if (s != RFD_OK) return s;
             RFD_HIP(ctx, hipGraphLaunch(ge, ctx->stream));
@@ -271,6 +313,7 @@ rfd_status rfd_batch_destroy(rfd_batch *b) {
     (void)hipFree(b->qseg);
     (void)hipFree(b->slots);
     (void)hipFree(b->progress);
+    (void)hipFree(b->flags64);
     (void)hipFree(b->items);
     (void)hipFree(b->bitems);
     (void)hipFree(b->count);
@@ -389,6 +432,7 @@ rfd_status rfd_batch_create(rfd_ctx *ctx, int64_t npairs, const double *const *P
     if ((e = hipMalloc(&b->qseg, sizeof(double) * qsofs)) != hipSuccess) return fail(e, "hipMalloc(qseg)");
     if ((e = hipMalloc(&b->slots, sizeof(double) * slofs)) != hipSuccess) return fail(e, "hipMalloc(slots)");
     if ((e = hipMalloc(&b->progress, sizeof(int) * bofs)) != hipSuccess) return fail(e, "hipMalloc(progress)");
+    if ((e = hipMalloc(&b->flags64, sizeof(unsigned long long) * bofs)) != hipSuccess) return fail(e, "hipMalloc(flags64)");
     if ((e = hipMalloc(&b->items, sizeof(int2) * bofs)) != hipSuccess) return fail(e, "hipMalloc(items)");
     if (!hbitems.empty()) {
         if ((e = hipMalloc(&b->bitems, sizeof(int2) * hbitems.size())) != hipSuccess) return fail(e, "hipMalloc(bound items)");

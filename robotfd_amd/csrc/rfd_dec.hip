@@ -8,6 +8,25 @@
 
 #include <algorithm>
 
+namespace rfd {
+
+// Flags, queue, result and counters of one windowed decision sweep.
+__global__ void dec_init(unsigned long long *flags, int nbands, int *queue, double *result, double sentinel,
+                         unsigned long long *evals, int *abort) {
+    const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int stride = (int)(gridDim.x * blockDim.x);
+    for (int k = tid; k < nbands; k += stride) flags[k] = flag64(0u, kLoUnknown);
+    if (tid == 0) {
+        *queue = 0;
+        result[0] = sentinel;
+        evals[0] = 0;
+        evals[1] = 0;
+        *abort = 0;
+    }
+}
+
+}  // namespace rfd
+
 namespace {
 
 template <int D, int R, int NP, bool VAR>
