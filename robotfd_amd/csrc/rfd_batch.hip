@@ -112,20 +112,24 @@ int batch_dec_grid_cfg(rfd_ctx *ctx) {
     return std::max(1, std::min(ctx->cu_count * per_cu, ctx->max_waves));
 }
 
-// Workgroups of a windowed batch decision round.
-int rfd_batch_dec_grid(rfd_ctx *ctx) {
-    switch (ctx->batch_dec) {
-        case 13: return batch_dec_grid_cfg<1, 3>(ctx);
-        case 16: return batch_dec_grid_cfg<1, 6>(ctx);
+// Workgroups of a windowed batch decision round at R rows per lane (six
+// producer waves).
+int rfd_batch_dec_grid(rfd_ctx *ctx, int rpl) {
+    switch (rpl) {
+        case 1: return batch_dec_grid_cfg<1, 6>(ctx);
+        case 3: return batch_dec_grid_cfg<3, 6>(ctx);
+        case 4: return batch_dec_grid_cfg<4, 6>(ctx);
         default: return batch_dec_grid_cfg<2, 6>(ctx);
     }
 }
 
 void launch_batch_dec(rfd_ctx *ctx, hipStream_t st, const rfd::BatchArgs &a) {
-    const int grid = rfd_batch_dec_grid(ctx);
-    switch (a.dec_win) {
-        case 13: hipLaunchKernelGGL((rfd::batch_dec<1, 3>), dim3(grid), dim3(64 * 5), 0, st, a); break;
-        case 16: hipLaunchKernelGGL((rfd::batch_dec<1, 6>), dim3(grid), dim3(64 * 8), 0, st, a); break;
+    const int rpl = a.dec_win / 10;
+    const int grid = rfd_batch_dec_grid(ctx, rpl);
+    switch (rpl) {
+        case 1: hipLaunchKernelGGL((rfd::batch_dec<1, 6>), dim3(grid), dim3(64 * 8), 0, st, a); break;
+        case 3: hipLaunchKernelGGL((rfd::batch_dec<3, 6>), dim3(grid), dim3(64 * 8), 0, st, a); break;
+        case 4: hipLaunchKernelGGL((rfd::batch_dec<4, 6>), dim3(grid), dim3(64 * 8), 0, st, a); break;
         default: hipLaunchKernelGGL((rfd::batch_dec<2, 6>), dim3(grid), dim3(64 * 8), 0, st, a); break;
     }
 }
@@ -159,10 +163,11 @@ rfd::BatchArgs make_args(rfd_batch *b, int rpl, const rfd_opts &o, int prune) {
     a.prune = prune;
     a.gate = o.vefd_gate;
     a.max_waves = b->ctx->max_waves;
-    a.dec_win = b->ctx->batch_dec;
-    // Speculation depth: windowed rounds count workgroups (several waves
-    // each, one band at a time) instead of single-wave bands.
-    if (a.dec_win) a.max_waves = rfd_batch_dec_grid(b->ctx) * b->ctx->batch_dec_budget;
+    // Decision rounds on the windowed sweep at the batch's rows per lane, six
+    // producer waves.  Speculation depth: they count workgroups (several
+    // waves each, one band at a time) instead of single-wave bands.
+    a.dec_win = b->ctx->batch_dec ? 10 * rpl + 6 : 0;
+    if (a.dec_win) a.max_waves = rfd_batch_dec_grid(b->ctx, rpl) * b->ctx->batch_dec_budget;
     a.max_depth = o.spec_depth >= 1 ? std::min(o.spec_depth, rfd::kMaxDepth) : rfd::kMaxDepth;
     a.watch = rfd::Watch{b->abort, b->ctx->spin_limit};
     return a;

@@ -688,10 +688,7 @@ rfd_status rfd_create(rfd_ctx **out, int device) {
         if (c == 0 || rfd_host::rfd_valid_dec_cfg(c)) ctx->dec_cfg = c;
     }
     if (const char *e = std::getenv("RFD_DEC_WGCU")) ctx->dec_wg_per_cu = std::max(0, std::atoi(e));
-    if (const char *e = std::getenv("RFD_BATCH_DEC")) {
-        const int c = std::atoi(e);
-        if (c == 0 || c == 13 || c == 16 || c == 26) ctx->batch_dec = c;
-    }
+    if (const char *e = std::getenv("RFD_BATCH_DEC")) ctx->batch_dec = std::atoi(e) != 0 ? 1 : 0;
     if (const char *e = std::getenv("RFD_BATCH_DEC_WGCU")) ctx->batch_dec_wgcu = std::max(0, std::atoi(e));
     if (const char *e = std::getenv("RFD_BATCH_DEC_BUDGET")) ctx->batch_dec_budget = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("RFD_ROWS")) {

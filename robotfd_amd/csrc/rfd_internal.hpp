@@ -29,7 +29,7 @@ struct rfd_ctx {
     int dec_cfg = -1;             // FP64 decisions: windowed sweep, 10 * rows per lane + producer waves; -1 = by size,
                                   // 0 = band_sweep (RFD_DEC)
     int dec_wg_per_cu = 1;        // windowed decision sweep: workgroups per CU, 0 = as many as fit (RFD_DEC_WGCU)
-    int batch_dec = 26;           // batch decision rounds on the windowed sweep (13, 16, 26); 0 = band_sweep (RFD_BATCH_DEC)
+    int batch_dec = 1;            // batch decision rounds on the windowed sweep at the batch's rows per lane; 0 = band_sweep (RFD_BATCH_DEC)
     int batch_dec_wgcu = 0;       // their workgroups per CU, 0 = as many as fit (RFD_BATCH_DEC_WGCU)
     int batch_dec_budget = 2;     // speculation: bands x nodes up to this many per workgroup (RFD_BATCH_DEC_BUDGET)
     int lds_per_cu = 0;           // bytes of LDS per CU

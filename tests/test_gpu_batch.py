@@ -164,12 +164,12 @@ def test_batch_bound_round_tall_pairs():
         ctx.close()
 
 
-@pytest.mark.parametrize("env", [dict(RFD_BATCH_DEC=0), dict(RFD_BATCH_DEC=13), dict(RFD_BATCH_DEC=16),
-                                 dict(RFD_BATCH_DEC=26, RFD_MAX_WAVES=2)])
-def test_batch_decision_kernels_match_oracle(env):
+@pytest.mark.parametrize("env,rows", [(dict(RFD_BATCH_DEC=0), 1), (dict(RFD_BATCH_DEC=0), 3),
+                                      (dict(RFD_MAX_WAVES=2), 2), (dict(RFD_MAX_WAVES=2), 4)])
+def test_batch_decision_kernels_match_oracle(env, rows):
     """Decision rounds on band_sweep (RFD_BATCH_DEC=0) and on the windowed
-    sweep at each configuration, also with two workgroups taking every item:
-    every pair bit-exact against the oracle, sequential and speculative."""
+    sweep with two workgroups taking every item: every pair bit-exact
+    against the oracle, sequential and speculative."""
     ctx = ctx_with(**env)
     try:
         pairs = mixed_pairs(24, seed=9)
@@ -178,7 +178,7 @@ def test_batch_decision_kernels_match_oracle(env):
         with Batch(pairs, ctx=ctx) as b:
             for depth in (1, 0):
                 print("batch decisions", env, "depth", depth, flush=True)
-                out = b.run(stop="rel", eps=1e-9, rows_per_lane=1, spec_depth=depth)
+                out = b.run(stop="rel", eps=1e-9, rows_per_lane=rows, spec_depth=depth)
                 for k, ref in enumerate(refs):
                     assert out["iterations"][k] == ref["iterations"], (env, depth, k)
                     assert out["distance"][k] == ref["distance"], (env, depth, k)
