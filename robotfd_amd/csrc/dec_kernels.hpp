@@ -651,14 +651,14 @@ __device__ inline void dec_producer(const DecArgs &a, const int band, const int 
 #pragma unroll
         for (int t = 0; t < R; ++t) {
             T rl, ru, tl, tu;
-            iv_ends<T, VAR>(Bv[2 * t], Dv[2 * t], Sv[2 * t], inv2b, rl, ru);
-            iv_ends<T, VAR>(Bv[2 * t + 1], Dv[2 * t + 1], Sv[2 * t + 1], inv2a[t], tl, tu);
+            // The chain's fused-form encoding: an empty interval's lower end
+            // is +inf.
+            const T dead_lb = VAR ? T(-1) : INF;
+            iv_ends<T, VAR>(Bv[2 * t], Dv[2 * t], Sv[2 * t], inv2b, rl, ru, dead_lb);
+            iv_ends<T, VAR>(Bv[2 * t + 1], Dv[2 * t + 1], Sv[2 * t + 1], inv2a[t], tl, tu, dead_lb);
             if constexpr (!VAR) {
-                // The chain's fused-form encoding: an empty interval's lower
-                // end is +inf; a padding row of the last band passes the
-                // value from below on unchanged and stays unreachable itself.
-                rl = (rl < T(0)) ? INF : rl;
-                tl = (tl < T(0)) ? INF : tl;
+                // A padding row of the last band passes the value from below
+                // on unchanged and stays unreachable itself.
                 if (pad_band && i0 + t >= rows) {
                     rl = T(0);
                     ru = T(2);

@@ -182,8 +182,11 @@ __device__ inline typename RootT<T>::type iv_root(T disc) {
     return Num<R>::sqrt(disc >= T(0) ? (R)disc : R(0));
 }
 
+// dead_lb: the lower end an empty interval gets (-1, the reference's; the
+// windowed decision sweep's chain takes +inf).
 template <typename T, bool TWO_PHASE>
-__device__ inline void iv_ends(T B, T disc, typename RootT<T>::type s, T inv2A, T &lb, T &ub) {
+__device__ inline void iv_ends(T B, T disc, typename RootT<T>::type s, T inv2A, T &lb, T &ub,
+                               const T dead_lb = T(-1)) {
     using N = Num<T>;
     using R = typename RootT<T>::type;
     const T ta = (T)((-(R)B - s) * (R)inv2A);
@@ -192,10 +195,10 @@ __device__ inline void iv_ends(T B, T disc, typename RootT<T>::type s, T inv2A, 
     // branch on disc >= 0, which keeps the cells' chains from interleaving.
     const bool valid = (disc >= T(0)) & !((tb < T(0)) | (ta > T(1)));
     if (TWO_PHASE) {
-        lb = valid ? ((ta < T(0)) ? T(0) : ta) : T(-1);
+        lb = valid ? ((ta < T(0)) ? T(0) : ta) : dead_lb;
         ub = valid ? ((tb > T(1)) ? T(1) : tb) : T(-2);
     } else {
-        lb = valid ? N::max(ta, T(0)) : T(-1);
+        lb = valid ? N::max(ta, T(0)) : dead_lb;
         ub = valid ? N::min(tb, T(1)) : T(-2);
     }
 }
