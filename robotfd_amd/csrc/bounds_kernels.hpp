@@ -74,9 +74,10 @@ __device__ inline void lds_put(int *p, int v) { __hip_atomic_store(p, v, __ATOMI
 // qring_cols(C))).
 constexpr int qring_cols(int C) { return 256 * C; }
 // Components per Q-ring column: q (3), and with five or more producers also
-// the column's segment terms q_j - q_{j-1} (3) and |q_j - q_{j-1}|^2, staged
-// once per chunk instead of recomputed by every producer wave every step.
-constexpr int qring_comps(int NPW) { return NPW >= 5 ? 7 : 3; }
+// the column's segment terms q_j - q_{j-1} (3), |q_j - q_{j-1}|^2 and the
+// weight test's limit (far_end_lim), staged once per chunk instead of
+// recomputed by every producer wave every step.
+constexpr int qring_comps(int NPW) { return NPW >= 5 ? 8 : 3; }
 
 // LDS plan of one workgroup.  The weight ring holds KS steps of 1536 R C bytes
 // (C columns per step), as many as fit (at most 16) beside the Q ring, the
@@ -123,50 +124,76 @@ __device__ inline double dpp_f64(double old, double src, int ctrl_shl) {
                                             (unsigned long long)(unsigned)lo));
 }
 
-// A cell's two point-segment weights: the squared distances of x1 to segment
-// [a1, a1 + b1] and of x2 to [a2, a2 + b2], given ap = x - a and l = |b|^2;
-// the operation order of point_seg2 (.cu:375-394), with the segment terms
-// hoisted.  The clamped parameter t = min(1, max(0, (ap . b) / l)) is 0 when
-// ap . b <= 0 (or NaN) and 1 when ap . b >= l, whatever the quotient rounds
-// to, so the correctly rounded divide runs only for projections strictly
-// inside the segment: about 0.2% of point-segment pairs on long random walks.
-// Both divides sit behind one branch: the branch, its masks and compares are
-// what a projection costs when it takes no divide, so sharing them is what
-// saves time (one branch per weight costs 2% at 200k x 200k).
+// |x - fma(t, b, a)|^2: a point-segment weight at clamped parameter t, the
+// operation order of point_seg2 (.cu:375-394) with the segment terms hoisted.
 template <int D>
-__device__ inline void seg_w_pair(const double *x1, const double *a1, const double *ap1, const double *b1,
-                                  const double l1, const double *x2, const double *a2, const double *ap2,
-                                  const double *b2, const double l2, double &w1, double &w2) {
+__device__ inline double seg_w_at(const double *x, const double *a, const double *b, const double t) {
+    double w = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double e = x[k] - __builtin_fma(t, b[k], a[k]);
+        w = __builtin_fma(e, e, w);
+    }
+    return w;
+}
+
+// A cell's two point-segment weights: the squared distances of x1 to segment
+// [a1, a1 + b1] and of x2 to [a2, a2 + b2], given ap = x - a and l = |b|^2,
+// and three squared distances the producer already holds: n1 = |x1 - a1|^2,
+// n2 = |x2 - a2|^2 and f = |x1 - e1|^2 = |x2 - e2|^2 (the DFD values of the
+// cell's neighbours and of the cell itself; e = the segment's far end), each
+// an FMA chain in the weight's own order.
+// The clamped parameter t = min(1, max(0, (ap . b) / l)) is 0 when ap . b <= 0
+// and 1 when ap . b >= l, whatever the quotient rounds to.
+//  - At t = 0 the projection fma(0, b, a) is a (a zero's sign aside, which no
+//    square sees), so the weight is n, bit for bit.
+//  - At t = 1 it is a + b rounded, which is the far end e exactly where that
+//    sum rounds back to it.  `lim` is l for such segments and +inf for the
+//    others, so p >= lim means the weight is f, bit for bit.
+//  - Otherwise (a projection strictly inside the segment, or t = 1 where a + b
+//    misses e) the weight is computed, with the correctly rounded divide for
+//    the interior.
+// The last case sits behind one branch for both weights; on the 200k-point
+// random walks a wave of one row takes it in about one step in six, and every
+// column's and row's far end is exact.
+template <int D>
+__device__ inline void seg_w_near(const double *x1, const double *a1, const double *ap1, const double *b1,
+                                  const double l1, const double lim1, const double n1, const double *x2,
+                                  const double *a2, const double *ap2, const double *b2, const double l2,
+                                  const double lim2, const double n2, const double f, double &w1, double &w2) {
     double p1 = 0.0, p2 = 0.0;
 #pragma unroll
     for (int k = 0; k < D; ++k) p1 = __builtin_fma(ap1[k], b1[k], p1);
 #pragma unroll
     for (int k = 0; k < D; ++k) p2 = __builtin_fma(ap2[k], b2[k], p2);
-    double t1 = (p1 > 0.0) ? 1.0 : 0.0, t2 = (p2 > 0.0) ? 1.0 : 0.0;
-    const bool in1 = p1 > 0.0 && p1 < l1, in2 = p2 > 0.0 && p2 < l2;
-    if (in1 || in2) {
+    const bool pos1 = p1 > 0.0, pos2 = p2 > 0.0;
+    w1 = pos1 ? f : n1;
+    w2 = pos2 ? f : n2;
+    const bool s1 = pos1 && p1 < lim1, s2 = pos2 && p2 < lim2;
+    if (s1 || s2) {
         asm volatile("");  // one branch for both, not selects
-        if (in1) {
+        if (s1) {
             asm volatile("");
-            t1 = __builtin_fmin(1.0, __builtin_fmax(0.0, p1 / l1));
+            const double t1 = (p1 < l1) ? __builtin_fmin(1.0, __builtin_fmax(0.0, p1 / l1)) : 1.0;
+            w1 = seg_w_at<D>(x1, a1, b1, t1);
         }
-        if (in2) {
+        if (s2) {
             asm volatile("");
-            t2 = __builtin_fmin(1.0, __builtin_fmax(0.0, p2 / l2));
+            const double t2 = (p2 < l2) ? __builtin_fmin(1.0, __builtin_fmax(0.0, p2 / l2)) : 1.0;
+            w2 = seg_w_at<D>(x2, a2, b2, t2);
         }
     }
-    w1 = 0.0;
-    w2 = 0.0;
+}
+
+// The weight test's limit for a segment from a to a + b (b = e - a rounded,
+// l = |b|^2): l where a + b rounds back to the far end e in every coordinate,
+// else +inf (see seg_w_near).
+template <int D>
+__device__ inline double far_end_lim(const double *a, const double *b, const double *e, const double l) {
+    bool ex = true;
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-        const double e = x1[k] - __builtin_fma(t1, b1[k], a1[k]);
-        w1 = __builtin_fma(e, e, w1);
-    }
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-        const double e = x2[k] - __builtin_fma(t2, b2[k], a2[k]);
-        w2 = __builtin_fma(e, e, w2);
-    }
+    for (int k = 0; k < D; ++k) ex = ex && (a[k] + b[k] == e[k]);
+    return ex ? l : __builtin_huge_val();
 }
 
 // min / max of doubles that are never NaN.  Where the translation unit is
@@ -190,6 +217,16 @@ static __device__ inline double dmax(double a, double b) {
     return r;
 }
 #endif
+
+// The lane index as a value the compiler cannot hoist out of the band loop:
+// each role derives its addresses from it inside its own band, instead of at
+// kernel entry, where the set-up of every role would stay live (and spill)
+// across the others' loops.
+__device__ inline int role_lane() {
+    int lane = (int)(threadIdx.x & 63);
+    asm volatile("" : "+v"(lane));
+    return lane;
+}
 
 // Row groups of the chain's software pipeline: NG groups of at most GMAX
 // rows, group g holding rows start(g) .. start(g + 1) - 1.  Three groups at
@@ -320,7 +357,7 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
     constexpr int SLOT = C * R * 3 * 64;  // doubles per ring slot
     constexpr int NONE = 0x3fffffff;      // no limit
     static_assert(NG >= 2, "the pipeline needs two row groups");
-    const int lane = (int)(threadIdx.x & 63);
+    const int lane = role_lane();
     const double big = (double)FLT_MAX;
     // Steps and columns in 32 bits (the launch requires n < 2^31 - 64 and, at
     // two columns per step, n even), so that the loop's bookkeeping stays on
@@ -563,7 +600,7 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
 template <int R, int NPW, int C>
 __device__ inline void bounds_courier(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C> &sh) {
     constexpr int KB = BoundsShared<R, NPW, C>::KB;
-    const int lane = (int)(threadIdx.x & 63);
+    const int lane = role_lane();
     const int cols = (int)a.cols;
     const double big = (double)FLT_MAX;
     const double *inD = a.slots + (long long)((band + 1) & 1) * 2 * cols;
@@ -661,16 +698,16 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
     constexpr int QR = qring_cols(C);
     constexpr int RP = NR;
     const int tfirst = ProducerRows<R, NPW>::first(pw);
-    const int lane = (int)(threadIdx.x & 63);
+    const int lane = role_lane();
     const long long rows = a.rows;
     const int cols = (int)a.cols;
     const double big = (double)FLT_MAX;
     const double *P = a.P, *Q = a.Q;
     const long long i0 = (long long)band * 64 * R + (long long)R * lane;
-    // Row terms: p_i, p_{i-1}, p_i - p_{i-1} and its squared length.  A row
-    // after the wave's first takes p_{i-1} from the row before it (both
-    // indices clamp to m - 1 alike in the padding).
-    double pi[RP][D], pm[RP][D], ab[RP][D], ab2[RP];
+    // Row terms: p_i, p_{i-1}, p_i - p_{i-1}, its squared length and the top
+    // weight test's limit.  A row after the wave's first takes p_{i-1} from
+    // the row before it (both indices clamp to m - 1 alike in the padding).
+    double pi[RP][D], pm[RP][D], ab[RP][D], ab2[RP], lim2[RP];
 #pragma unroll
     for (int k = 0; k < RP; ++k) {
         const long long i = i0 + tfirst + k;
@@ -686,6 +723,7 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
             acc = __builtin_fma(ab[k][c], ab[k][c], acc);
         }
         ab2[k] = acc;
+        lim2[k] = far_end_lim<D>(pm[k], ab[k], pi[k], acc);
     }
     // Q ring.  Lane l reads columns C(s - l) .. C(s - l) + C - 1 at step s.
     // Every producer wave stores each chunk itself (the same values): the
@@ -695,22 +733,23 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
     // steps ahead of the chain), so no wave still reads the chunk 4C chunks
     // back, whose slots the stored chunk takes, and none reads a chunk before
     // its own store of it (a wave's LDS operations complete in order).
-    constexpr bool QT = qring_comps(NPW) == 7;
+    constexpr bool QT = qring_comps(NPW) == 8;
     // Stores column c's entries: q_c, and with the segment terms q_c -
-    // q_{c-1} and its squared length (the operations of the column terms
-    // below, so the same values), q_{c-1} being `qm`.
+    // q_{c-1}, its squared length and the weight test's limit (the operations
+    // of the column terms below, so the same values), q_{c-1} being `qm`.
     auto put_column = [&](const int idx, const double(&qc)[D], const double(&qm)[D]) __attribute__((always_inline)) {
 #pragma unroll
         for (int k = 0; k < D; ++k) qring[k * QR + idx] = qc[k];
         if constexpr (QT) {
-            double c2 = 0.0;
+            double c2 = 0.0, cab[D];
 #pragma unroll
             for (int k = 0; k < D; ++k) {
-                const double cab = qc[k] - qm[k];
-                c2 = __builtin_fma(cab, cab, c2);
-                qring[(3 + k) * QR + idx] = cab;
+                cab[k] = qc[k] - qm[k];
+                c2 = __builtin_fma(cab[k], cab[k], c2);
+                qring[(3 + k) * QR + idx] = cab[k];
             }
             qring[6 * QR + idx] = c2;
+            qring[7 * QR + idx] = far_end_lim<D>(qm, cab, qc, c2);
         }
     };
     double qn[D];
@@ -733,24 +772,27 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
     const int nsteps = band_steps<R, C>(a, band);
     int cons_seen = 0;
     int slot = 0;
-    // Carried from the column before: q_{j-1}, and p_i - q_{j-1} of each row,
+    // Carried from the column before: q_{j-1}, and per row p_i - q_{j-1},
     // which is the right weight's p_i - a term (the same operands and
-    // operation as the DFD difference one column earlier, so the same value).
+    // operation as the DFD difference one column earlier, so the same value),
+    // and that DFD value |p_i - q_{j-1}|^2, the right weight at t = 0.
     // Values at j <= 0 feed only weights that the boundary rules below
     // overwrite.
-    double qA[D], qB[D], xA[RP][D], xB[RP][D];
+    double qA[D], qB[D], xA[RP][D], xB[RP][D], dA[RP], dB[RP];
 #pragma unroll
     for (int c = 0; c < D; ++c) qA[c] = 0.0;
 #pragma unroll
-    for (int k = 0; k < RP; ++k)
+    for (int k = 0; k < RP; ++k) {
+        dA[k] = 0.0;
 #pragma unroll
         for (int c = 0; c < D; ++c) xA[k][c] = 0.0;
+    }
 
     // q of the wave's next column, loaded a column ahead (a wave's columns
     // follow each other: j, j + 1, ...).  At j < 0 it reads ring slots not yet
     // holding Q; those values feed only weights that the boundary rules
     // overwrite.
-    double qp[D], cp[D], c2p = 0.0;
+    double qp[D], cp[D], c2p = 0.0, lp = 0.0;
     auto prefetch = [&](const int qi) __attribute__((always_inline)) {
 #pragma unroll
         for (int k = 0; k < D; ++k) qp[k] = qring[k * QR + qi];
@@ -758,27 +800,31 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
 #pragma unroll
             for (int k = 0; k < D; ++k) cp[k] = qring[(3 + k) * QR + qi];
             c2p = qring[6 * QR + qi];
+            lp = qring[7 * QR + qi];
         }
     };
     prefetch((C * (0 - lane)) & (QR - 1));
 
     // Weights of column j for this wave's rows into w (one column of a slot).
     auto column = [&](const int j, const double(&qa)[D], double(&qb)[D], const double(&xp)[RP][D],
-                      double(&xn)[RP][D], double *w, const bool edge) __attribute__((always_inline)) {
+                      const double(&dp)[RP], double(&xn)[RP][D], double(&dn)[RP], double *w, const bool edge)
+                      __attribute__((always_inline)) {
         // Column terms of segment [q_{j-1}, q_j].
-        double cab[D], cab2 = 0.0;
+        double cab[D], cab2 = 0.0, clim;
 #pragma unroll
         for (int k = 0; k < D; ++k) qb[k] = qp[k];
         if constexpr (QT) {
 #pragma unroll
             for (int k = 0; k < D; ++k) cab[k] = cp[k];
             cab2 = c2p;
+            clim = lp;
         } else {
 #pragma unroll
             for (int k = 0; k < D; ++k) {
                 cab[k] = qb[k] - qa[k];
                 cab2 = __builtin_fma(cab[k], cab[k], cab2);
             }
+            clim = far_end_lim<D>(qa, cab, qb, cab2);
         }
         prefetch((j + 1) & (QR - 1));
 #pragma unroll
@@ -797,8 +843,20 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
             double dt[D];
 #pragma unroll
             for (int c = 0; c < D; ++c) dt[c] = (k == 0) ? qb[c] - pm[k][c] : -xn[k - 1][c];
+            // |q_j - p_{i-1}|^2, the top weight at t = 0: for a row after the
+            // wave's first, the row before's DFD value (its terms are dt's
+            // negations, whose squares are the same).
+            double nt = 0.0;
+            if (k == 0) {
+#pragma unroll
+                for (int c = 0; c < D; ++c) nt = __builtin_fma(dt[c], dt[c], nt);
+            } else {
+                nt = dn[k - 1];
+            }
             double wr, wt;
-            seg_w_pair<D>(pi[k], qa, xp[k], cab, cab2, qb, pm[k], dt, ab[k], ab2[k], wr, wt);
+            seg_w_near<D>(pi[k], qa, xp[k], cab, cab2, clim, dp[k], qb, pm[k], dt, ab[k], ab2[k], lim2[k], nt, d,
+                          wr, wt);
+            dn[k] = d;
             w[(t * 3 + 0) * 64] = d;
             w[(t * 3 + 1) * 64] = wr;
             w[(t * 3 + 2) * 64] = wt;
@@ -876,8 +934,8 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
             double *w = wring + slot * (C * R * 3 * 64) + lane;
             const bool edge = s < 64 || edge_band;
             const int j = C * (s - lane);
-            column(j, qA, qB, xA, xB, w, edge);
-            column(j + 1, qB, qA, xB, xA, w + R * 3 * 64, edge);
+            column(j, qA, qB, xA, dA, xB, dB, w, edge);
+            column(j + 1, qB, qA, xB, dB, xA, dA, w + R * 3 * 64, edge);
             step_end(s);
         }
     } else {
@@ -902,11 +960,11 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         // rows): those steps run a copy of the loop that checks them, the
         // others one without the check.
         auto one = [&](auto edge_c, const int s, const double(&qa)[D], double(&qb)[D], const double(&xp)[RP][D],
-                       double(&xn)[RP][D], const bool first, const bool last, const int ahead)
-                       __attribute__((always_inline)) {
+                       const double(&dp)[RP], double(&xn)[RP][D], double(&dn)[RP], const bool first, const bool last,
+                       const int ahead) __attribute__((always_inline)) {
             if (first) step_begin(s, last ? 0 : ahead);
             double *w = wring + slot * (R * 3 * 64) + lane;
-            column(s - lane, qa, qb, xp, xn, w, decltype(edge_c)::value);
+            column(s - lane, qa, qb, xp, dp, xn, dn, w, decltype(edge_c)::value);
             if (last) {
                 step_end(s);
             } else {
@@ -917,18 +975,18 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         auto steps_to = [&](auto e, const int end) __attribute__((always_inline)) {
             if constexpr (QUADS) {
                 for (; s + 3 < end; s += 4) {
-                    one(e, s, qA, qB, xA, xB, true, false, 3);
-                    one(e, s + 1, qB, qA, xB, xA, false, false, 0);
-                    one(e, s + 2, qA, qB, xA, xB, false, false, 0);
-                    one(e, s + 3, qB, qA, xB, xA, false, true, 0);
+                    one(e, s, qA, qB, xA, dA, xB, dB, true, false, 3);
+                    one(e, s + 1, qB, qA, xB, dB, xA, dA, false, false, 0);
+                    one(e, s + 2, qA, qB, xA, dA, xB, dB, false, false, 0);
+                    one(e, s + 3, qB, qA, xB, dB, xA, dA, false, true, 0);
                 }
             }
             for (; s + 1 < end; s += 2) {
-                one(e, s, qA, qB, xA, xB, true, false, 1);
-                one(e, s + 1, qB, qA, xB, xA, false, true, 0);
+                one(e, s, qA, qB, xA, dA, xB, dB, true, false, 1);
+                one(e, s + 1, qB, qA, xB, dB, xA, dA, false, true, 0);
             }
             if (s < end) {
-                one(e, s, qA, qB, xA, xB, true, true, 0);
+                one(e, s, qA, qB, xA, dA, xB, dB, true, true, 0);
                 ++s;
             }
         };
@@ -945,7 +1003,11 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
 template <int D, int R, int NPW, int C = 1, int DS = D>
 __device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C> &sh) {
     constexpr int KS = BoundsRing<R, NPW, C>::KS;
-    const int wave = (int)(threadIdx.x >> 6);
+    // The wave index as a value the compiler cannot hoist out of the band
+    // loop, so that no role's set-up is computed at kernel entry and held
+    // (spilled) across the other roles' loops.
+    int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    asm volatile("" : "+s"(wave));
     // Waves of a workgroup are dealt to the CU's four SIMDs in turn, so wave 4
     // shares a SIMD with wave 0: that is the courier, which mostly sleeps, and
     // the chain's SIMD keeps its issue slots for the chain.
