@@ -137,57 +137,30 @@ __device__ inline double seg_w_at(const double *x, const double *a, const double
     return w;
 }
 
-// A cell's two point-segment weights: the squared distances of x1 to segment
-// [a1, a1 + b1] and of x2 to [a2, a2 + b2], given ap = x - a and l = |b|^2,
-// and three squared distances the producer already holds: n1 = |x1 - a1|^2,
-// n2 = |x2 - a2|^2 and f = |x1 - e1|^2 = |x2 - e2|^2 (the DFD values of the
-// cell's neighbours and of the cell itself; e = the segment's far end), each
-// an FMA chain in the weight's own order.
-// The clamped parameter t = min(1, max(0, (ap . b) / l)) is 0 when ap . b <= 0
-// and 1 when ap . b >= l, whatever the quotient rounds to.
-//  - At t = 0 the projection fma(0, b, a) is a (a zero's sign aside, which no
-//    square sees), so the weight is n, bit for bit.
-//  - At t = 1 it is a + b rounded, which is the far end e exactly where that
-//    sum rounds back to it.  `lim` is l for such segments and +inf for the
-//    others, so p >= lim means the weight is f, bit for bit.
-//  - Otherwise (a projection strictly inside the segment, or t = 1 where a + b
-//    misses e) the weight is computed, with the correctly rounded divide for
-//    the interior.
-// The last case sits behind one branch for both weights; on the 200k-point
-// random walks a wave of one row takes it in about one step in six, and every
-// column's and row's far end is exact.
-template <int D>
-__device__ inline void seg_w_near(const double *x1, const double *a1, const double *ap1, const double *b1,
-                                  const double l1, const double lim1, const double n1, const double *x2,
-                                  const double *a2, const double *ap2, const double *b2, const double l2,
-                                  const double lim2, const double n2, const double f, double &w1, double &w2) {
-    double p1 = 0.0, p2 = 0.0;
-#pragma unroll
-    for (int k = 0; k < D; ++k) p1 = __builtin_fma(ap1[k], b1[k], p1);
-#pragma unroll
-    for (int k = 0; k < D; ++k) p2 = __builtin_fma(ap2[k], b2[k], p2);
-    const bool pos1 = p1 > 0.0, pos2 = p2 > 0.0;
-    w1 = pos1 ? f : n1;
-    w2 = pos2 ? f : n2;
-    const bool s1 = pos1 && p1 < lim1, s2 = pos2 && p2 < lim2;
-    if (s1 || s2) {
-        asm volatile("");  // one branch for both, not selects
-        if (s1) {
-            asm volatile("");
-            const double t1 = (p1 < l1) ? __builtin_fmin(1.0, __builtin_fmax(0.0, p1 / l1)) : 1.0;
-            w1 = seg_w_at<D>(x1, a1, b1, t1);
-        }
-        if (s2) {
-            asm volatile("");
-            const double t2 = (p2 < l2) ? __builtin_fmin(1.0, __builtin_fmax(0.0, p2 / l2)) : 1.0;
-            w2 = seg_w_at<D>(x2, a2, b2, t2);
-        }
-    }
+// A VE-FD weight is the squared distance of a point x to a segment [a, a + b]
+// (b = e - a rounded, e the far end, l = |b|^2, ap = x - a), with clamped
+// parameter t = min(1, max(0, (ap . b) / l)): 0 when ap . b <= 0 and 1 when
+// ap . b >= l, whatever the quotient rounds to.  The producer already holds
+// |x - a|^2 and |x - e|^2 as DFD values (of the cell and its neighbours, each
+// an FMA chain in the weight's own order), so:
+//  - at t = 0 the projection fma(0, b, a) is a (a zero's sign aside, which no
+//    square sees), and the weight is |x - a|^2, bit for bit;
+//  - at t = 1 it is a + b rounded, which is e exactly where that sum rounds
+//    back to it; `lim` is l for such segments and +inf for the others, so
+//    ap . b >= lim means the weight is |x - e|^2, bit for bit;
+//  - otherwise (a projection strictly inside the segment, or t = 1 where
+//    a + b misses e) the weight is computed, by seg_w_at at seg_t_slow's t,
+//    with the correctly rounded divide for the interior.
+// The last case sits behind one branch per wave step (every row's two
+// weights); on the 200k-point random walks a wave of one row takes it in
+// about one step in six, and every column's and row's far end is exact.
+__device__ inline double seg_t_slow(const double p, const double l) {
+    return (p < l) ? __builtin_fmin(1.0, __builtin_fmax(0.0, p / l)) : 1.0;
 }
 
 // The weight test's limit for a segment from a to a + b (b = e - a rounded,
 // l = |b|^2): l where a + b rounds back to the far end e in every coordinate,
-// else +inf (see seg_w_near).
+// else +inf (see seg_t_slow).
 template <int D>
 __device__ inline double far_end_lim(const double *a, const double *b, const double *e, const double l) {
     bool ex = true;
@@ -827,9 +800,12 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
             clim = far_end_lim<D>(qa, cab, qb, cab2);
         }
         prefetch((j + 1) & (QR - 1));
+        // Per row: the projections' dot products, the weights at a clamped
+        // parameter (seg_t_slow's rule) and whether a weight needs computing.
+        double p1[RP], p2[RP], dt[RP][D], wr[RP], wt[RP];
+        bool s1[RP], s2[RP], slow = false;
 #pragma unroll
         for (int k = 0; k < RP; ++k) {
-            const int t = tfirst + k;
             double d = 0.0;
 #pragma unroll
             for (int c = 0; c < D; ++c) {
@@ -840,26 +816,54 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
             // -(p_{i-1} - q_j), the row before's difference (round to nearest
             // is symmetric; a zero's sign cannot change the projection's sum,
             // which starts at +0).
-            double dt[D];
 #pragma unroll
-            for (int c = 0; c < D; ++c) dt[c] = (k == 0) ? qb[c] - pm[k][c] : -xn[k - 1][c];
+            for (int c = 0; c < D; ++c) dt[k][c] = (k == 0) ? qb[c] - pm[k][c] : -xn[k - 1][c];
             // |q_j - p_{i-1}|^2, the top weight at t = 0: for a row after the
             // wave's first, the row before's DFD value (its terms are dt's
             // negations, whose squares are the same).
             double nt = 0.0;
             if (k == 0) {
 #pragma unroll
-                for (int c = 0; c < D; ++c) nt = __builtin_fma(dt[c], dt[c], nt);
+                for (int c = 0; c < D; ++c) nt = __builtin_fma(dt[k][c], dt[k][c], nt);
             } else {
                 nt = dn[k - 1];
             }
-            double wr, wt;
-            seg_w_near<D>(pi[k], qa, xp[k], cab, cab2, clim, dp[k], qb, pm[k], dt, ab[k], ab2[k], lim2[k], nt, d,
-                          wr, wt);
+            // Right weight: p_i to [q_{j-1}, q_j] (ap = p_i - q_{j-1}); top
+            // weight: q_j to [p_{i-1}, p_i] (ap = q_j - p_{i-1}).
+            double a1 = 0.0, a2 = 0.0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) a1 = __builtin_fma(xp[k][c], cab[c], a1);
+#pragma unroll
+            for (int c = 0; c < D; ++c) a2 = __builtin_fma(dt[k][c], ab[k][c], a2);
+            p1[k] = a1;
+            p2[k] = a2;
+            wr[k] = a1 > 0.0 ? d : dp[k];
+            wt[k] = a2 > 0.0 ? d : nt;
+            s1[k] = a1 > 0.0 && a1 < clim;
+            s2[k] = a2 > 0.0 && a2 < lim2[k];
+            slow = slow || s1[k] || s2[k];
             dn[k] = d;
-            w[(t * 3 + 0) * 64] = d;
-            w[(t * 3 + 1) * 64] = wr;
-            w[(t * 3 + 2) * 64] = wt;
+        }
+        if (slow) {
+            asm volatile("");  // one branch for the wave's rows, not selects
+#pragma unroll
+            for (int k = 0; k < RP; ++k) {
+                if (s1[k]) {
+                    asm volatile("");
+                    wr[k] = seg_w_at<D>(pi[k], qa, cab, seg_t_slow(p1[k], cab2));
+                }
+                if (s2[k]) {
+                    asm volatile("");
+                    wt[k] = seg_w_at<D>(qb, pm[k], ab[k], seg_t_slow(p2[k], ab2[k]));
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RP; ++k) {
+            const int t = tfirst + k;
+            w[(t * 3 + 0) * 64] = dn[k];
+            w[(t * 3 + 1) * 64] = wr[k];
+            w[(t * 3 + 2) * 64] = wt[k];
         }
         if (edge) {
             // Boundary and padding cells overwrite their VE-FD weights.  A real
