@@ -572,11 +572,11 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
 // [cols].
 // Every flag poll and column access is a device-scope access that L2 does not
 // keep, so each is memory traffic.  Bands of 13 rows per lane, whose steps are
-// long, therefore publish at least PUB = 16 columns at a time (the band's last
-// columns excepted) and poll the predecessor only once every column it has
-// published is in the ring and the chain has fewer than 32 incoming columns
-// left: at 200k x 200k the sweep's time is unchanged and its memory traffic
-// falls from 5.9 to 3.2 GB.  Shorter steps publish every column as soon as it
+// long, therefore publish whole pieces of PUB = 16 columns (the band's last
+// columns excepted), copy them in such pieces, and poll the predecessor only
+// once every column it has published is in the ring and the chain has fewer
+// than 32 incoming columns left: at 200k x 200k the sweep's time is unchanged and its memory traffic
+// falls from 5.9 to 2.5 GB.  Shorter steps publish every column as soon as it
 // is ready, since there the batches lengthen the pipeline's fill (5% at 7 rows
 // per lane on a 100k x 120k pair).
 template <int R, int NPW, int C>
@@ -609,6 +609,7 @@ __device__ inline void bounds_courier(const BoundsArgs &a, const int band, Bound
                 int n = known - in_next;
                 n = n < room ? n : room;
                 n = n < 64 ? n : 64;
+                if (BATCH && in_next + n < known) n &= ~(PUB - 1);  // whole 128-byte pieces
                 if (n > 0) {
                     __atomic_signal_fence(__ATOMIC_SEQ_CST);
                     if (lane < n) {
@@ -638,6 +639,10 @@ __device__ inline void bounds_courier(const BoundsArgs &a, const int band, Bound
             // Lane 63 writes columns C (s - 63) .. at step s.
             int n = C * (done - 63) - out_next;
             n = n < 64 ? n : 64;
+            // Whole 16-column (128-byte) pieces, the band's last columns
+            // excepted: a store that covers part of a line makes the memory
+            // side read the line to merge it.
+            if (BATCH && out_next + n < cols) n &= ~(PUB - 1);
             if (n >= PUB || (n > 0 && out_next + n == cols)) {
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
                 if (lane < n) {
