@@ -575,10 +575,10 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
 // long, therefore publish whole pieces of PUB = 16 columns (the band's last
 // columns excepted), copy them in such pieces, and poll the predecessor only
 // once every column it has published is in the ring and the chain has fewer
-// than 32 incoming columns left: at 200k x 200k the sweep's time is unchanged and its memory traffic
-// falls from 5.9 to 2.5 GB.  Shorter steps publish every column as soon as it
-// is ready, since there the batches lengthen the pipeline's fill (5% at 7 rows
-// per lane on a 100k x 120k pair).
+// than 32 incoming columns left: at 200k x 200k the sweep's time is unchanged
+// and its memory traffic falls from 5.9 to 2.5 GB.  Shorter steps publish
+// every column as soon as it is ready, since there the batches lengthen the
+// pipeline's fill (5% at 7 rows per lane on a 100k x 120k pair).
 template <int R, int NPW, int C>
 __device__ inline void bounds_courier(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C> &sh) {
     constexpr int KB = BoundsShared<R, NPW, C>::KB;
