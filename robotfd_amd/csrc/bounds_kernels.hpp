@@ -76,10 +76,12 @@ constexpr int qring_cols(int C) { return 256 * C; }
 // Components per Q-ring column: q (3), and with five or more producers also
 // the column's segment terms q_j - q_{j-1} (3), |q_j - q_{j-1}|^2 and the
 // weight test's limit (far_end_lim), staged once per chunk instead of
-// recomputed by every producer wave every step.
-constexpr int qring_comps(int NPW) { return NPW >= 5 ? 8 : 3; }
+// recomputed by every producer wave every step.  WK: weights per cell, 3 for
+// the fused sweep (DFD distance^2, VE-FD right and top weights), 1 for the
+// DFD-only sweep, which needs q alone.
+constexpr int qring_comps(int NPW, int WK = 3) { return (WK == 3 && NPW >= 5) ? 8 : 3; }
 
-// LDS plan of one workgroup.  The weight ring holds KS steps of 1536 R C bytes
+// LDS plan of one workgroup.  The weight ring holds KS steps of 512 WK R C bytes
 // (C columns per step), as many as fit (at most 16) beside the Q ring, the
 // boundary rings and the hand-off words: within 160 KB for the configurations
 // with five or more producers (one workgroup per CU), within a third of that
@@ -87,14 +89,14 @@ constexpr int qring_comps(int NPW) { return NPW >= 5 ? 8 : 3; }
 // of KS steps, one per ring slot; the boundary rings hold KB positions, a
 // multiple of C KS (so that a trip's positions never wrap), at least 128
 // (64 with few producers, whose steps are short).
-template <int R, int NPW, int C = 1> struct BoundsRing {
+template <int R, int NPW, int C = 1, int WK = 3> struct BoundsRing {
     static constexpr int kb(int ks) {
         const int min_cols = (NPW < 5) ? 64 : 128;
         return ((min_cols + C * ks - 1) / (C * ks)) * (C * ks);
     }
     static constexpr int bytes(int ks) {
-        return ks * 1536 * R * C + qring_comps(NPW) * qring_cols(C) * 8 + 2 * 2 * kb(ks) * 8 + 2 * (64 + C * ks) * 8 +
-               2 * 64 * 4 + 16 * 4 + 64;
+        return ks * 512 * WK * R * C + qring_comps(NPW, WK) * qring_cols(C) * 8 + 2 * 2 * kb(ks) * 8 +
+               2 * (64 + C * ks) * 8 + 2 * 64 * 4 + 16 * 4 + 64;
     }
     static constexpr int cap = (NPW < 5) ? 163840 / 3 : 163840;
     static constexpr int pick() {
@@ -215,29 +217,34 @@ template <int R, int C, int G = 0> struct ChainPlan {
 
 // Weights of group g for the C columns of one step, issued as one batch of LDS
 // loads.  A slot holds [column][row][d, wr, wt][lane].
-template <int R, int C, int G, int g>
-__device__ inline void chain_load(const double *w, double (&wv)[C][ChainPlan<R, C, G>::GMAX][3]) {
+template <int R, int C, int G, int g, int WK>
+__device__ inline void chain_load(const double *w, double (&wv)[C][ChainPlan<R, C, G>::GMAX][WK]) {
     using PL = ChainPlan<R, C, G>;
 #pragma unroll
     for (int c = 0; c < C; ++c)
 #pragma unroll
         for (int i = 0; i < PL::size(g); ++i)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) wv[c][i][k] = w[(c * R + PL::start(g) + i) * 3 * 64 + k * 64];
+            for (int k = 0; k < WK; ++k) wv[c][i][k] = w[(c * R + PL::start(g) + i) * WK * 64 + k * 64];
 }
 
 // Orders a group's results before any later memory access: the next loads
 // into the group's registers must not be hoisted above the compute that
 // frees them.
-template <int R, int C, int G, int g>
+template <int R, int C, int G, int g, int WK>
 __device__ inline void chain_done(const double (&dbot)[R], const double (&vbot)[R], const double (&dl)[C],
                                   const double (&vl)[C]) {
     using PL = ChainPlan<R, C, G>;
 #pragma unroll
-    for (int c = 0; c < C; ++c) asm volatile("" ::"v"(dl[c]), "v"(vl[c]) : "memory");
+    for (int c = 0; c < C; ++c) {
+        if constexpr (WK == 3) asm volatile("" ::"v"(dl[c]), "v"(vl[c]) : "memory");
+        else asm volatile("" ::"v"(dl[c]) : "memory");
+    }
 #pragma unroll
-    for (int i = 0; i < PL::size(g); ++i)
-        asm volatile("" ::"v"(dbot[PL::start(g) + i]), "v"(vbot[PL::start(g) + i]) : "memory");
+    for (int i = 0; i < PL::size(g); ++i) {
+        if constexpr (WK == 3) asm volatile("" ::"v"(dbot[PL::start(g) + i]), "v"(vbot[PL::start(g) + i]) : "memory");
+        else asm volatile("" ::"v"(dbot[PL::start(g) + i]) : "memory");
+    }
 }
 
 // Group g's rows of one chain lane for the C columns of one step, column by
@@ -246,8 +253,9 @@ __device__ inline void chain_done(const double (&dbot)[R], const double (&vbot)[
 // own values from the previous column:
 //   DFD   (.cu:458-641)  top = max(d, min(dl, dbot)),  dl' = min(top, dbot)
 //   VE-FD (.cu:200-432)  in = min(vl, vbot),  T = max(wt, in),  vl' = R = max(wr, in)
-template <int R, int C, int G, int g>
-__device__ inline void chain_rows(const double (&wv)[C][ChainPlan<R, C, G>::GMAX][3], double (&dbot)[R],
+// (VE-FD only where the cell carries its weights, WK = 3).
+template <int R, int C, int G, int g, int WK>
+__device__ inline void chain_rows(const double (&wv)[C][ChainPlan<R, C, G>::GMAX][WK], double (&dbot)[R],
                                   double (&vbot)[R], double (&dl)[C], double (&vl)[C]) {
     using PL = ChainPlan<R, C, G>;
     // One wait for the whole group (its loads were issued a group earlier),
@@ -257,22 +265,25 @@ __device__ inline void chain_rows(const double (&wv)[C][ChainPlan<R, C, G>::GMAX
 #pragma unroll
         for (int i = 0; i < PL::size(g); ++i)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) asm volatile("" ::"v"(wv[c][i][k]));
+            for (int k = 0; k < WK; ++k) asm volatile("" ::"v"(wv[c][i][k]));
 #pragma unroll
     for (int c = 0; c < C; ++c)
 #pragma unroll
         for (int i = 0; i < PL::size(g); ++i) {
             const int t = PL::start(g) + i;
-            const double d = wv[c][i][0], wr = wv[c][i][1], wt = wv[c][i][2];
+            const double d = wv[c][i][0];
             // dl' = min(top, dbot) = min(max(d, dl), dbot) (min distributes
             // over max, and these doubles are never NaN): two operations on
             // the serial path instead of three, and no copy of the old dbot.
             const double dn = dmin(dmax(d, dl[c]), dbot[t]);
             dbot[t] = dmax(d, dmin(dl[c], dbot[t]));
             dl[c] = dn;
-            const double vin = dmin(vl[c], vbot[t]);
-            vbot[t] = dmax(wt, vin);
-            vl[c] = dmax(wr, vin);
+            if constexpr (WK == 3) {
+                const double wr = wv[c][i][1], wt = wv[c][i][2];
+                const double vin = dmin(vl[c], vbot[t]);
+                vbot[t] = dmax(wt, vin);
+                vl[c] = dmax(wr, vin);
+            }
         }
 }
 
@@ -291,9 +302,9 @@ __device__ inline void static_for(F &&f) {
 // at position (c + 63 C) mod KB, so that a step's positions follow its index
 // and a trip's never wrap.  Lanes 0-62 of the chain store their outputs to
 // slots of their own (odummy), so that the chain's store needs no lane mask.
-template <int R, int NPW, int C = 1> struct BoundsShared {
-    static constexpr int KS = BoundsRing<R, NPW, C>::KS;
-    static constexpr int KB = BoundsRing<R, NPW, C>::KB;
+template <int R, int NPW, int C = 1, int WK = 3> struct BoundsShared {
+    static constexpr int KS = BoundsRing<R, NPW, C, WK>::KS;
+    static constexpr int KB = BoundsRing<R, NPW, C, WK>::KB;
     // The small arrays first, so that their addresses fit a DS instruction's
     // 16-bit offset.
     alignas(16) int sprod[16];  // steps each producer has written
@@ -305,8 +316,8 @@ template <int R, int NPW, int C = 1> struct BoundsShared {
     alignas(16) double inring[2 * KB];
     alignas(16) double outring[2 * KB];
     alignas(16) double odummy[2 * (64 + C * KS)];
-    alignas(16) double qring[qring_comps(NPW) * qring_cols(C)];
-    alignas(16) double wring[KS * C * R * 3 * 64];
+    alignas(16) double qring[qring_comps(NPW, WK) * qring_cols(C)];
+    alignas(16) double wring[KS * C * R * WK * 64];
 };
 
 // Steps of band `band`.  Lane l evaluates columns C (s - l) .. C (s - l) + C - 1
@@ -320,14 +331,14 @@ __device__ inline int band_steps(const BoundsArgs &a, const int band) {
     return (toff >= 0 && toff < 64LL * R) ? cols / C + (int)(toff / R) : cols / C + 63;
 }
 
-template <int R, int NPW, int C>
-__device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C> &sh) {
+template <int R, int NPW, int C, int WK>
+__device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C, WK> &sh) {
     constexpr int GP = (NPW + 2 > 12) ? 3 : 0;  // four waves per SIMD: 128 VGPRs
     using PL = ChainPlan<R, C, GP>;
     constexpr int NG = PL::NG;
-    constexpr int KS = BoundsShared<R, NPW, C>::KS;
-    constexpr int KB = BoundsShared<R, NPW, C>::KB;
-    constexpr int SLOT = C * R * 3 * 64;  // doubles per ring slot
+    constexpr int KS = BoundsShared<R, NPW, C, WK>::KS;
+    constexpr int KB = BoundsShared<R, NPW, C, WK>::KB;
+    constexpr int SLOT = C * R * WK * 64;  // doubles per ring slot
     constexpr int NONE = 0x3fffffff;      // no limit
     static_assert(NG >= 2, "the pipeline needs two row groups");
     const int lane = role_lane();
@@ -419,13 +430,13 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
     // weights occupy registers at a time.  Loads are issued on every path
     // (the last step reads a slot and ring positions nobody uses), so that
     // the compiler's LDS counter waits stay exact.
-    double w[NG][C][PL::GMAX][3];
+    double w[NG][C][PL::GMAX][WK];
     double bD[C], bV[C];  // the next step's input from below (lane 0)
     auto load_in = [&](const double *p) __attribute__((always_inline)) {
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             bD[c] = p[2 * c];
-            bV[c] = p[2 * c + 1];
+            if constexpr (WK == 3) bV[c] = p[2 * c + 1];
         }
     };
     wait_for(-1);
@@ -455,7 +466,7 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
             return wb[kk / NB] + (kk % NB) * SLOT;
         }
     };
-    chain_load<R, C, GP, 0>(slot_ptr(std::integral_constant<int, 0>{}), w[0]);
+    chain_load<R, C, GP, 0, WK>(slot_ptr(std::integral_constant<int, 0>{}), w[0]);
     load_in(sh.inring);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     // Steps finished, in a vector register: the store of the count then needs
@@ -478,12 +489,12 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             dl[c] = dpp_f64(bD[c], dout[c], 0);
-            vl[c] = dpp_f64(bV[c], vout[c], 0);
+            if constexpr (WK == 3) vl[c] = dpp_f64(bV[c], vout[c], 0);
         }
         static_for<0, NG>([&](auto G) __attribute__((always_inline)) {
             constexpr int g = decltype(G)::value;
             if constexpr (g + 1 < NG) {
-                chain_load<R, C, GP, g + 1>(wcur, w[g + 1]);
+                chain_load<R, C, GP, g + 1, WK>(wcur, w[g + 1]);
                 if constexpr (g + 2 == NG) {
                     // The step's last loads from its slot are issued, and a
                     // wave's LDS operations complete in order: producers may
@@ -497,19 +508,21 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
                     lim = limit();
                     lrel = lim - s0;
                 }
-                chain_load<R, C, GP, 0>(wnext, w[0]);
+                chain_load<R, C, GP, 0, WK>(wnext, w[0]);
                 load_in(in_next);
             }
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            chain_rows<R, C, GP, g>(w[g], dbot, vbot, dl, vl);
-            chain_done<R, C, GP, g>(dbot, vbot, dl, vl);
+            chain_rows<R, C, GP, g, WK>(w[g], dbot, vbot, dl, vl);
+            chain_done<R, C, GP, g, WK>(dbot, vbot, dl, vl);
         });
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             dout[c] = dl[c];
-            vout[c] = vl[c];
             ow[2 * c] = dout[c];
-            ow[2 * c + 1] = vout[c];
+            if constexpr (WK == 3) {
+                vout[c] = vl[c];
+                ow[2 * c + 1] = vout[c];
+            }
         }
         // The outputs are in the ring and step s0 + k's weights and input in
         // registers: the step is finished.
@@ -551,11 +564,13 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
             topD = (tl == t) ? dbot[t] : topD;
             topV = (tl == t) ? vbot[t] : topV;
         }
-        const double resV = dmin(topV, vout[C - 1]);
         a.result[0] = __builtin_sqrt(topD);
         a.result[1] = topD;
-        a.result[2] = __builtin_sqrt(resV);
-        a.result[3] = resV;
+        if constexpr (WK == 3) {
+            const double resV = dmin(topV, vout[C - 1]);
+            a.result[2] = __builtin_sqrt(resV);
+            a.result[3] = resV;
+        }
     }
     if (a.tlog && lane == 0) {
         a.tlog[4 * band + 2] = wall_clock64();
@@ -579,9 +594,9 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
 // and its memory traffic falls from 5.9 to 2.5 GB.  Shorter steps publish
 // every column as soon as it is ready, since there the batches lengthen the
 // pipeline's fill (5% at 7 rows per lane on a 100k x 120k pair).
-template <int R, int NPW, int C>
-__device__ inline void bounds_courier(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C> &sh) {
-    constexpr int KB = BoundsShared<R, NPW, C>::KB;
+template <int R, int NPW, int C, int WK>
+__device__ inline void bounds_courier(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C, WK> &sh) {
+    constexpr int KB = BoundsShared<R, NPW, C, WK>::KB;
     constexpr bool BATCH = R >= 13;
     constexpr int PUB = BATCH ? 16 : 1;
     const int lane = role_lane();
@@ -614,17 +629,16 @@ __device__ inline void bounds_courier(const BoundsArgs &a, const int band, Bound
                     __atomic_signal_fence(__ATOMIC_SEQ_CST);
                     if (lane < n) {
                         const int c = in_next + lane;
-                        double vD, vV;
+                        double vD, vV = big;
                         if (first) {
                             vD = (c == 0) ? 0.0 : big;
-                            vV = big;
                         } else {
                             vD = column_load<SYNC_SCOPED>(inD + c);
-                            vV = column_load<SYNC_SCOPED>(inV + c);
+                            if constexpr (WK == 3) vV = column_load<SYNC_SCOPED>(inV + c);
                         }
                         const int p = c % KB;
                         sh.inring[2 * p] = vD;
-                        sh.inring[2 * p + 1] = vV;
+                        if constexpr (WK == 3) sh.inring[2 * p + 1] = vV;
                     }
                     __atomic_signal_fence(__ATOMIC_SEQ_CST);
                     __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the ring writes are done
@@ -649,7 +663,7 @@ __device__ inline void bounds_courier(const BoundsArgs &a, const int band, Bound
                     const int c = out_next + lane;
                     const int p = (c + 63 * C) % KB;
                     column_store<SYNC_SCOPED>(outD + c, sh.outring[2 * p]);
-                    column_store<SYNC_SCOPED>(outV + c, sh.outring[2 * p + 1]);
+                    if constexpr (WK == 3) column_store<SYNC_SCOPED>(outV + c, sh.outring[2 * p + 1]);
                 }
                 __builtin_amdgcn_wave_barrier();
                 if (lane == 0) flag_store<SYNC_SCOPED>(&a.progress[band], out_next + n);
@@ -683,7 +697,7 @@ template <int R, int NPW> struct ProducerRows {
 // `cons`: the chain's count of steps whose ring slot it has read.  Points are
 // DS doubles apart, of which the first D are read (the batch stores every
 // curve 3D: its 2D pairs run D = 2 over DS = 3, whose extra coordinate is 0).
-template <int D, int R, int NPW, int KS, int C, int NR, int DS = D>
+template <int D, int R, int NPW, int KS, int C, int NR, int DS = D, int WK = 3>
 __device__ inline void bounds_producer(const BoundsArgs &a, const int band, const int pw, double *wring,
                                        double *qring, int *prod, int *cons) {
     constexpr int QR = qring_cols(C);
@@ -724,7 +738,7 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
     // steps ahead of the chain), so no wave still reads the chunk 4C chunks
     // back, whose slots the stored chunk takes, and none reads a chunk before
     // its own store of it (a wave's LDS operations complete in order).
-    constexpr bool QT = qring_comps(NPW) == 8;
+    constexpr bool QT = qring_comps(NPW, WK) == 8;
     // Stores column c's entries: q_c, and with the segment terms q_c -
     // q_{c-1}, its squared length and the weight test's limit (the operations
     // of the column terms below, so the same values), q_{c-1} being `qm`.
@@ -800,6 +814,24 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
     auto column = [&](const int j, const double(&qa)[D], double(&qb)[D], const double(&xp)[RP][D],
                       const double(&dp)[RP], double(&xn)[RP][D], double(&dn)[RP], double *w, const bool edge)
                       __attribute__((always_inline)) {
+        if constexpr (WK == 1) {
+            // The DFD-only sweep: each cell's distance^2, and +big before a
+            // lane's first column (the row's initial value).
+#pragma unroll
+            for (int k = 0; k < D; ++k) qb[k] = qp[k];
+            prefetch((j + 1) & (QR - 1));
+#pragma unroll
+            for (int k = 0; k < RP; ++k) {
+                double d = 0.0;
+#pragma unroll
+                for (int c = 0; c < D; ++c) {
+                    const double x = pi[k][c] - qb[c];
+                    d = __builtin_fma(x, x, d);
+                }
+                w[(tfirst + k) * 64] = (edge && j < 0) ? big : d;
+            }
+            return;
+        }
         // Column terms of segment [q_{j-1}, q_j].
         double cab[D], cab2 = 0.0, clim;
 #pragma unroll
@@ -953,11 +985,11 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         // one column to the next instead of being copied.
         for (int s = 0; s < nsteps; ++s) {
             step_begin(s, 0);
-            double *w = wring + slot * (C * R * 3 * 64) + lane;
+            double *w = wring + slot * (C * R * WK * 64) + lane;
             const bool edge = s < 64 || edge_band;
             const int j = C * (s - lane);
             column(j, qA, qB, xA, dA, xB, dB, w, edge);
-            column(j + 1, qB, qA, xB, dB, xA, dA, w + R * 3 * 64, edge);
+            column(j + 1, qB, qA, xB, dB, xA, dA, w + R * WK * 64, edge);
             step_end(s);
         }
     } else {
@@ -985,7 +1017,7 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
                        const double(&dp)[RP], double(&xn)[RP][D], double(&dn)[RP], const bool first, const bool last,
                        const int ahead) __attribute__((always_inline)) {
             if (first) step_begin(s, last ? 0 : ahead);
-            double *w = wring + slot * (R * 3 * 64) + lane;
+            double *w = wring + slot * (R * WK * 64) + lane;
             column(s - lane, qa, qb, xp, dp, xn, dn, w, decltype(edge_c)::value);
             if (last) {
                 step_end(s);
@@ -1021,10 +1053,11 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
 
 // One band of one pair, by every wave of the workgroup.  The caller has
 // cleared the sync words and ends with a barrier before reusing the LDS.
-// C = 2 (two columns per step) requires n even.  DS: the points' stride.
-template <int D, int R, int NPW, int C = 1, int DS = D>
-__device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C> &sh) {
-    constexpr int KS = BoundsRing<R, NPW, C>::KS;
+// C = 2 (two columns per step) requires n even.  DS: the points' stride.  WK:
+// 3 for DFD and VE-FD, 1 for DFD alone.
+template <int D, int R, int NPW, int C = 1, int DS = D, int WK = 3>
+__device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C, WK> &sh) {
+    constexpr int KS = BoundsRing<R, NPW, C, WK>::KS;
     // The wave index as a value the compiler cannot hoist out of the band
     // loop, so that no role's set-up is computed at kernel entry and held
     // (spilled) across the other roles' loops.
@@ -1036,21 +1069,22 @@ __device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsSh
     constexpr int cw = (NPW + 2 > 4) ? 4 : NPW + 1;
     constexpr int RP = (R + NPW - 1) / NPW;
     if (wave == 0) {
-        bounds_chain<R, NPW, C>(a, band, sh);
+        bounds_chain<R, NPW, C, WK>(a, band, sh);
     } else if (wave == cw) {
-        bounds_courier<R, NPW, C>(a, band, sh);
+        bounds_courier<R, NPW, C, WK>(a, band, sh);
     } else {
         const int pw = wave - 1 - (wave > cw ? 1 : 0);
         if (RP * NPW == R || pw < R - (RP - 1) * NPW) {
-            bounds_producer<D, R, NPW, KS, C, RP, DS>(a, band, pw, sh.wring, sh.qring, sh.sprod, &sh.wfree[0]);
+            bounds_producer<D, R, NPW, KS, C, RP, DS, WK>(a, band, pw, sh.wring, sh.qring, sh.sprod, &sh.wfree[0]);
         } else if constexpr (RP > 1) {
-            bounds_producer<D, R, NPW, KS, C, RP - 1, DS>(a, band, pw, sh.wring, sh.qring, sh.sprod, &sh.wfree[0]);
+            bounds_producer<D, R, NPW, KS, C, RP - 1, DS, WK>(a, band, pw, sh.wring, sh.qring, sh.sprod,
+                                                                &sh.wfree[0]);
         }
     }
 }
 
-template <int R, int NPW, int C>
-__device__ inline void bounds_clear(BoundsShared<R, NPW, C> &sh) {
+template <int R, int NPW, int C, int WK>
+__device__ inline void bounds_clear(BoundsShared<R, NPW, C, WK> &sh) {
     for (int w = 0; w < NPW; ++w) sh.sprod[w] = 0;
     sh.cons[0] = 0;
     sh.wfree[0] = 0;
@@ -1058,11 +1092,13 @@ __device__ inline void bounds_clear(BoundsShared<R, NPW, C> &sh) {
     sh.out_done = 0;
 }
 
-template <int D, int R, int NPW, int C>
+// WK = 3: DFD and VE-FD (result[0..3]); WK = 1: DFD alone (result[0..1]).
+template <int D, int R, int NPW, int C, int WK = 3>
 __global__ __launch_bounds__(64 * (NPW + 2)) void bounds_sweep(BoundsArgs a) {
     static_assert(NPW >= 1 && NPW <= R, "one to R producer waves");
     static_assert(C == 1 || C == 2, "one or two columns per step");
-    __shared__ BoundsShared<R, NPW, C> sh;
+    static_assert(WK == 1 || WK == 3, "DFD alone, or DFD and VE-FD");
+    __shared__ BoundsShared<R, NPW, C, WK> sh;
     for (;;) {
         if (threadIdx.x == 0) {
             sh.item = atomicAdd(a.queue, 1);
@@ -1071,7 +1107,7 @@ __global__ __launch_bounds__(64 * (NPW + 2)) void bounds_sweep(BoundsArgs a) {
         __syncthreads();
         const int band = sh.item;
         if (band >= a.nbands) break;
-        bounds_band<D, R, NPW, C>(a, band, sh);
+        bounds_band<D, R, NPW, C, D, WK>(a, band, sh);
         __syncthreads();
     }
 }

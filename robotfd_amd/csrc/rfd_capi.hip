@@ -136,6 +136,8 @@ namespace {
 using rfd_host::set_err;
 using rfd_host::endpoint_bound;
 using rfd_host::rfd_launch_bounds;
+using rfd_host::rfd_launch_dfd;
+using rfd_host::rfd_valid_dfd_cfg;
 using rfd_host::rfd_valid_batch_bounds_cfg;
 using rfd_host::rfd_valid_bounds_cfg;
 
@@ -317,8 +319,11 @@ rfd_status run_sweep(rfd_session *s, const T *P, const T *Q, const T *qseg, doub
     return RFD_OK;
 }
 
-// One fused bound sweep of a session (m, n >= 2): DFD, DFD^2 and VE-FD.
-rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, double *vefd) {
+// One fused bound sweep of a session (m, n >= 2): DFD, DFD^2 and VE-FD; with
+// `ve` false its DFD-only form (DFD and DFD^2; *ran false when the
+// configuration has no DFD-only form, and the caller runs band_sweep).
+rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, double *vefd, bool ve = true,
+                      bool *ran = nullptr) {
     rfd_ctx *ctx = s->ctx;
     // Configuration: RFD_BOUNDS if set; otherwise one band per CU with the
     // fewest rows per lane (7, 10 or 13) that cover the pair (7 producer
@@ -338,6 +343,11 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
         cfg = need <= 4 ? 42 : need <= 7 ? 77 : need <= 10 ? 107 : 1310;
     }
     if (cfg >= 20000 && (s->n & 1)) cfg -= 20000;
+    if (ran) *ran = true;
+    if (!ve && !rfd_valid_dfd_cfg(cfg)) {
+        if (ran) *ran = false;
+        return RFD_OK;
+    }
     const int base = cfg % 10000;
     const int R = base >= 1000 ? base / 100 : base / 10;  // 10 R + NPW, or 100 R + NPW for NPW >= 10
     const int64_t rows = s->m, cols = s->n;
@@ -367,7 +377,11 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
                        sentinel, s->evals, s->abort);
     RFD_HIP(ctx, hipGetLastError());
     int grid = 0;
-    RFD_HIP(ctx, rfd_launch_bounds(ctx, s->dims, cfg, nb, a, &grid));
+    if (ve) {
+        RFD_HIP(ctx, rfd_launch_bounds(ctx, s->dims, cfg, nb, a, &grid));
+    } else {
+        RFD_HIP(ctx, rfd_launch_dfd(ctx, s->dims, cfg, nb, a, &grid));
+    }
     RFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     double hres[4] = {0, 0, 0, 0};
     int habort = 0;
@@ -376,18 +390,20 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
     RFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0;
     RFD_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    // Cells of both bound sweeps: m * n DFD cells and (m-1)(n-1) VE-FD cells.
-    const double cells = (double)rows * (double)cols + (double)(rows - 1) * (double)(cols - 1);
-    ctx->stats.launches[RFD_K_BOUNDS64] += 1;
-    ctx->stats.ms[RFD_K_BOUNDS64] += ms;
-    ctx->stats.cells[RFD_K_BOUNDS64] += cells;
-    ctx->stats.evaluated[RFD_K_BOUNDS64] += cells;
+    // Cells of both bound sweeps: m * n DFD cells and (m-1)(n-1) VE-FD cells;
+    // the DFD-only form counts as a DFD sweep.
+    const double cells = (double)rows * (double)cols + (ve ? (double)(rows - 1) * (double)(cols - 1) : 0.0);
+    const int kind = ve ? RFD_K_BOUNDS64 : RFD_K_DFD;
+    ctx->stats.launches[kind] += 1;
+    ctx->stats.ms[kind] += ms;
+    ctx->stats.cells[kind] += cells;
+    ctx->stats.evaluated[kind] += cells;
     if (tlog) {
         std::vector<unsigned long long> h_t((size_t)(4 * nb));
         RFD_HIP(ctx, hipMemcpy(h_t.data(), tlog, sizeof(unsigned long long) * 4 * nb, hipMemcpyDeviceToHost));
         RFD_HIP(ctx, hipFree(tlog));
         if (FILE *f = std::fopen(ctx->tlog_path, "ab")) {
-            long long hdr[6] = {(long long)RFD_K_BOUNDS64, (long long)rows, (long long)cols, (long long)nb,
+            long long hdr[6] = {(long long)kind, (long long)rows, (long long)cols, (long long)nb,
                                 (long long)grid, (long long)cfg};
             std::fwrite(hdr, sizeof(hdr), 1, f);
             std::fwrite(h_t.data(), sizeof(unsigned long long), h_t.size(), f);
@@ -398,13 +414,13 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
         set_err(ctx, "fused bound sweep: a hand-off wait exceeded %u polls and gave up", ctx->spin_limit);
         return RFD_ERR_STATE;
     }
-    if (hres[0] == sentinel || hres[2] == sentinel) {
+    if (hres[0] == sentinel || (ve && hres[2] == sentinel)) {
         set_err(ctx, "fused bound sweep wrote no terminal value");
         return RFD_ERR_STATE;
     }
     *dfd = hres[0];
     *dfd_sq = hres[1];
-    *vefd = hres[2];
+    if (ve) *vefd = hres[2];
     return RFD_OK;
 }
 
@@ -799,8 +815,13 @@ rfd_status rfd_session_bounds(rfd_session *s, double *dfd, double *vefd, int run
         if (st == RFD_OK) s->dfd_sq = F;
         return st;
     }
-    rfd_status st = run_sweep<rfd::OP_DFD, double>(s, s->dP, s->dQ, nullptr, 0.0, RFD_K_DFD, dfd,
-                                                   INFINITY, &F);
+    // DFD alone: the fused sweep's DFD-only form where it has one, else the
+    // band sweep.
+    bool ran = false;
+    rfd_status st = RFD_OK;
+    if (ctx->fused && s->m >= 2 && s->n >= 2) st = run_bounds(s, 0.0, dfd, &F, nullptr, false, &ran);
+    if (st != RFD_OK) return st;
+    if (!ran) st = run_sweep<rfd::OP_DFD, double>(s, s->dP, s->dQ, nullptr, 0.0, RFD_K_DFD, dfd, INFINITY, &F);
     if (st != RFD_OK) return st;
     s->dfd_sq = F;
     if (run_vefd && s->m >= 2 && s->n >= 2) {

@@ -61,6 +61,25 @@ def check_pair(ctx, P, Q):
     assert ve == O.vefd(P, Q, lb)
 
 
+# The DFD-only form (one weight per cell, no VE-FD), which computes the DFD of
+# the FP32 and pruned bound paths, in every configuration it is built in.
+@pytest.mark.parametrize("cfg", [42, 77, 107, 1310])
+@pytest.mark.parametrize("dims", [2, 3])
+def test_dfd_only_configs_bit_exact(contexts, cfg, dims):
+    ctx = contexts(RFD_BOUNDS=cfg)
+    for m, n in SHAPES:
+        rng = np.random.default_rng(m * 1000 + n + dims + cfg + 7)
+        P, Q = walk(rng, m, dims), walk(rng, n, dims)
+        say("dfd only", cfg, dims, (m, n))
+        lb = O.endpoint_lb(P, Q)
+        with Intersection(P, Q, ctx=ctx) as s:
+            dfd, _ = s.Bound_call_gpu(lb, run_vefd=False)
+            assert dfd == O.dfd(P, Q)
+            # The FP32 bound path: DFD by the same form, VE-FD in FP32.
+            dfd32, ve32 = s.Bound_call_gpu(lb, fp32=True)
+            assert dfd32 == dfd and ve32 == O.vefd(P, Q, lb, fp32=True)
+
+
 # 20000 + c: the same configuration at two columns per step (even n; odd n
 # runs the one-column kernel).
 @pytest.mark.parametrize("cfg", [21, 42, 62, 63, 72, 73, 77, 83, 84, 107, 136, 137, 1012, 1310, 1313,

@@ -52,6 +52,8 @@ def test_long_pair_matches_oracle(gpu, gold, key):
     with Intersection(P, Q, ctx=gpu) as s:
         dfd, ve = s.Bound_call_gpu(g["endpoint_lb"])
         assert (dfd, ve) == (g["dfd"], g["vefd"])
+        say(key, "DFD-only fused sweep")
+        assert s.Bound_call_gpu(g["endpoint_lb"], run_vefd=False)[0] == g["dfd"]
         for r, ub in g["decisions"]:
             assert s.decide_ub(r) == ub, r
         d = s.distance(stop="rel", eps=1e-9)
@@ -64,6 +66,7 @@ def test_long_pair_matches_oracle(gpu, gold, key):
             assert err <= 1e-9
             assert mx["err_bound"] >= err
             assert mx["lb"] <= g["distance"] <= mx["ub"]
+            assert mx["dfd"] == g["dfd"]
 
 
 @pytest.mark.parametrize("key", ["c2", "c3"])
