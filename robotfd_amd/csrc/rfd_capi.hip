@@ -559,11 +559,16 @@ rfd_status refined_ub(rfd_session *s, int levels, double *ub) {
     t.abort = s->abort;
     rfd_status st = refine_curve(ctx, s->dP, s->m, s->dims, levels, &t.dP, &t.m);
     if (st == RFD_OK) st = refine_curve(ctx, s->dQ, s->n, s->dims, levels, &t.dQ, &t.n);
-    if (st == RFD_OK && hipMalloc(&t.slots, sizeof(double) * 2 * t.n) != hipSuccess) st = RFD_ERR_NOMEM;
+    // Boundary columns for the fused sweep's layout ([2 buffers][2 kinds][n]).
+    if (st == RFD_OK && hipMalloc(&t.slots, sizeof(double) * 4 * t.n) != hipSuccess) st = RFD_ERR_NOMEM;
     if (st == RFD_OK && hipMalloc(&t.progress, sizeof(int) * (nbands_for(t.m) + 1)) != hipSuccess) st = RFD_ERR_NOMEM;
     if (st == RFD_OK) {
+        // The fused sweep's DFD-only form where it has one, else the band sweep.
         double F = 0;
-        st = run_sweep<rfd::OP_DFD, double>(&t, t.dP, t.dQ, nullptr, 0.0, RFD_K_DFD, ub, INFINITY, &F);
+        bool ran = false;
+        if (ctx->fused && t.m >= 2 && t.n >= 2) st = run_bounds(&t, 0.0, ub, &F, nullptr, false, &ran);
+        if (st == RFD_OK && !ran)
+            st = run_sweep<rfd::OP_DFD, double>(&t, t.dP, t.dQ, nullptr, 0.0, RFD_K_DFD, ub, INFINITY, &F);
     } else if (st == RFD_ERR_NOMEM) {
         set_err(ctx, "no device memory for the refined curves (%lld x %lld points)", (long long)t.m,
                 (long long)t.n);
