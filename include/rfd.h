@@ -102,7 +102,8 @@ typedef struct {
 typedef enum {
     RFD_K_DFD = 0, RFD_K_VEFD64 = 1, RFD_K_DEC64 = 2, RFD_K_VEFD32 = 3, RFD_K_DEC32 = 4,
     RFD_K_BATCH = 5, RFD_K_MLP = 6, RFD_K_OTHER = 7,
-    RFD_K_BOUNDS64 = 8     /* fused DFD + VE-FD sweep (FP64); cells = m*n + (m-1)*(n-1) */
+    RFD_K_BOUNDS64 = 8,    /* fused DFD + VE-FD sweep (FP64); cells = m*n + (m-1)*(n-1) */
+    RFD_K_BOUNDS_MIXED = 9 /* fused sweep, DFD in FP64 and VE-FD in FP32; cells as RFD_K_BOUNDS64 */
 } rfd_kernel_kind;
 
 typedef struct rfd_ctx rfd_ctx;

@@ -24,9 +24,9 @@ PREC_FP64, PREC_MIXED = 0, 1
 DECIDE_FUSED, DECIDE_TWO_PHASE, DECIDE_REFERENCE = 0, 1, 2
 DECIDE_FORMS = {"fused": DECIDE_FUSED, "two-phase": DECIDE_TWO_PHASE, "reference": DECIDE_REFERENCE}
 
-K_DFD, K_VEFD64, K_DEC64, K_VEFD32, K_DEC32, K_BATCH, K_MLP, K_OTHER, K_BOUNDS64 = range(9)
+K_DFD, K_VEFD64, K_DEC64, K_VEFD32, K_DEC32, K_BATCH, K_MLP, K_OTHER, K_BOUNDS64, K_BOUNDS_MIXED = range(10)
 KERNEL_NAMES = ["dfd", "vefd_f64", "decide_f64", "vefd_f32", "decide_f32", "batch", "mlp", "other",
-                "bounds_f64", "kind9", "kind10", "kind11"]
+                "bounds_f64", "bounds_mixed", "kind10", "kind11"]
 N_KINDS = 12
 
 

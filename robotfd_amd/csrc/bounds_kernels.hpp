@@ -76,12 +76,21 @@ constexpr int qring_cols(int C) { return 256 * C; }
 // Components per Q-ring column: q (3), and with five or more producers also
 // the column's segment terms q_j - q_{j-1} (3), |q_j - q_{j-1}|^2 and the
 // weight test's limit (far_end_lim), staged once per chunk instead of
-// recomputed by every producer wave every step.  WK: weights per cell, 3 for
-// the fused sweep (DFD distance^2, VE-FD right and top weights), 1 for the
-// DFD-only sweep, which needs q alone.
-constexpr int qring_comps(int NPW, int WK = 3) { return (WK == 3 && NPW >= 5) ? 8 : 3; }
+// recomputed by every producer wave every step.
+// Sweep modes: DFD and VE-FD in FP64 (three weights per cell: DFD distance^2,
+// VE-FD right and top weights); DFD alone (one weight per cell, q alone in the
+// ring); DFD in FP64 and VE-FD in FP32 (three weights per cell, the VE-FD ones
+// FP32 values held in doubles, so the chain's min / max, which are exact, give
+// the FP32 recursion's values; the ring holds q and, packed two to a double,
+// q, its segment terms and test limit rounded to FP32).
+constexpr int kModeF64 = 3, kModeDfd = 1, kModeV32 = 2;
+constexpr int mode_weights(int mode) { return mode == kModeDfd ? 1 : 3; }
+constexpr bool mode_ve(int mode) { return mode != kModeDfd; }
+constexpr int qring_comps(int NPW, int mode = kModeF64) {
+    return NPW < 5 ? 3 : mode == kModeF64 ? 8 : mode == kModeV32 ? 7 : 3;
+}
 
-// LDS plan of one workgroup.  The weight ring holds KS steps of 512 WK R C bytes
+// LDS plan of one workgroup.  The weight ring holds KS steps of 512 W R C bytes (W weights per cell)
 // (C columns per step), as many as fit (at most 16) beside the Q ring, the
 // boundary rings and the hand-off words: within 160 KB for the configurations
 // with five or more producers (one workgroup per CU), within a third of that
@@ -89,13 +98,13 @@ constexpr int qring_comps(int NPW, int WK = 3) { return (WK == 3 && NPW >= 5) ? 
 // of KS steps, one per ring slot; the boundary rings hold KB positions, a
 // multiple of C KS (so that a trip's positions never wrap), at least 128
 // (64 with few producers, whose steps are short).
-template <int R, int NPW, int C = 1, int WK = 3> struct BoundsRing {
+template <int R, int NPW, int C = 1, int MODE = kModeF64> struct BoundsRing {
     static constexpr int kb(int ks) {
         const int min_cols = (NPW < 5) ? 64 : 128;
         return ((min_cols + C * ks - 1) / (C * ks)) * (C * ks);
     }
     static constexpr int bytes(int ks) {
-        return ks * 512 * WK * R * C + qring_comps(NPW, WK) * qring_cols(C) * 8 + 2 * 2 * kb(ks) * 8 +
+        return ks * 512 * mode_weights(MODE) * R * C + qring_comps(NPW, MODE) * qring_cols(C) * 8 + 2 * 2 * kb(ks) * 8 +
                2 * (64 + C * ks) * 8 + 2 * 64 * 4 + 16 * 4 + 64;
     }
     static constexpr int cap = (NPW < 5) ? 163840 / 3 : 163840;
@@ -126,15 +135,29 @@ __device__ inline double dpp_f64(double old, double src, int ctrl_shl) {
                                             (unsigned long long)(unsigned)lo));
 }
 
+// Two FP32 values in one double's bits (low, high), for the Q ring's FP32
+// entries.
+__device__ inline double pack_f2(float lo, float hi) {
+    return __longlong_as_double((long long)(((unsigned long long)__float_as_uint(hi) << 32) |
+                                            (unsigned long long)__float_as_uint(lo)));
+}
+__device__ inline float unpack_lo(double v) { return __uint_as_float((unsigned)(unsigned long long)__double_as_longlong(v)); }
+__device__ inline float unpack_hi(double v) {
+    return __uint_as_float((unsigned)((unsigned long long)__double_as_longlong(v) >> 32));
+}
+
 // |x - fma(t, b, a)|^2: a point-segment weight at clamped parameter t, the
-// operation order of point_seg2 (.cu:375-394) with the segment terms hoisted.
-template <int D>
-__device__ inline double seg_w_at(const double *x, const double *a, const double *b, const double t) {
-    double w = 0.0;
+// operation order of point_seg2 (.cu:375-394) with the segment terms hoisted;
+// T is double, or float for the FP32 VE-FD.
+__device__ inline double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ inline float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <int D, typename T>
+__device__ inline T seg_w_at(const T *x, const T *a, const T *b, const T t) {
+    T w = T(0);
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-        const double e = x[k] - __builtin_fma(t, b[k], a[k]);
-        w = __builtin_fma(e, e, w);
+        const T e = x[k] - fma_t(t, b[k], a[k]);
+        w = fma_t(e, e, w);
     }
     return w;
 }
@@ -159,16 +182,19 @@ __device__ inline double seg_w_at(const double *x, const double *a, const double
 __device__ inline double seg_t_slow(const double p, const double l) {
     return (p < l) ? __builtin_fmin(1.0, __builtin_fmax(0.0, p / l)) : 1.0;
 }
+__device__ inline float seg_t_slow(const float p, const float l) {
+    return (p < l) ? __builtin_fminf(1.0f, __builtin_fmaxf(0.0f, p / l)) : 1.0f;
+}
 
 // The weight test's limit for a segment from a to a + b (b = e - a rounded,
 // l = |b|^2): l where a + b rounds back to the far end e in every coordinate,
 // else +inf (see seg_t_slow).
-template <int D>
-__device__ inline double far_end_lim(const double *a, const double *b, const double *e, const double l) {
+template <int D, typename T>
+__device__ inline T far_end_lim(const T *a, const T *b, const T *e, const T l) {
     bool ex = true;
 #pragma unroll
     for (int k = 0; k < D; ++k) ex = ex && (a[k] + b[k] == e[k]);
-    return ex ? l : __builtin_huge_val();
+    return ex ? l : T(__builtin_huge_val());
 }
 
 // min / max of doubles that are never NaN.  Where the translation unit is
@@ -217,32 +243,33 @@ template <int R, int C, int G = 0> struct ChainPlan {
 
 // Weights of group g for the C columns of one step, issued as one batch of LDS
 // loads.  A slot holds [column][row][d, wr, wt][lane].
-template <int R, int C, int G, int g, int WK>
-__device__ inline void chain_load(const double *w, double (&wv)[C][ChainPlan<R, C, G>::GMAX][WK]) {
+template <int R, int C, int G, int g, int MODE>
+__device__ inline void chain_load(const double *w, double (&wv)[C][ChainPlan<R, C, G>::GMAX][mode_weights(MODE)]) {
     using PL = ChainPlan<R, C, G>;
 #pragma unroll
     for (int c = 0; c < C; ++c)
 #pragma unroll
         for (int i = 0; i < PL::size(g); ++i)
 #pragma unroll
-            for (int k = 0; k < WK; ++k) wv[c][i][k] = w[(c * R + PL::start(g) + i) * WK * 64 + k * 64];
+            for (int k = 0; k < mode_weights(MODE); ++k)
+                wv[c][i][k] = w[(c * R + PL::start(g) + i) * mode_weights(MODE) * 64 + k * 64];
 }
 
 // Orders a group's results before any later memory access: the next loads
 // into the group's registers must not be hoisted above the compute that
 // frees them.
-template <int R, int C, int G, int g, int WK>
+template <int R, int C, int G, int g, int MODE>
 __device__ inline void chain_done(const double (&dbot)[R], const double (&vbot)[R], const double (&dl)[C],
                                   const double (&vl)[C]) {
     using PL = ChainPlan<R, C, G>;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-        if constexpr (WK == 3) asm volatile("" ::"v"(dl[c]), "v"(vl[c]) : "memory");
+        if constexpr (mode_ve(MODE)) asm volatile("" ::"v"(dl[c]), "v"(vl[c]) : "memory");
         else asm volatile("" ::"v"(dl[c]) : "memory");
     }
 #pragma unroll
     for (int i = 0; i < PL::size(g); ++i) {
-        if constexpr (WK == 3) asm volatile("" ::"v"(dbot[PL::start(g) + i]), "v"(vbot[PL::start(g) + i]) : "memory");
+        if constexpr (mode_ve(MODE)) asm volatile("" ::"v"(dbot[PL::start(g) + i]), "v"(vbot[PL::start(g) + i]) : "memory");
         else asm volatile("" ::"v"(dbot[PL::start(g) + i]) : "memory");
     }
 }
@@ -253,9 +280,9 @@ __device__ inline void chain_done(const double (&dbot)[R], const double (&vbot)[
 // own values from the previous column:
 //   DFD   (.cu:458-641)  top = max(d, min(dl, dbot)),  dl' = min(top, dbot)
 //   VE-FD (.cu:200-432)  in = min(vl, vbot),  T = max(wt, in),  vl' = R = max(wr, in)
-// (VE-FD only where the cell carries its weights, WK = 3).
-template <int R, int C, int G, int g, int WK>
-__device__ inline void chain_rows(const double (&wv)[C][ChainPlan<R, C, G>::GMAX][WK], double (&dbot)[R],
+// (VE-FD only where the cell carries its weights).
+template <int R, int C, int G, int g, int MODE>
+__device__ inline void chain_rows(const double (&wv)[C][ChainPlan<R, C, G>::GMAX][mode_weights(MODE)], double (&dbot)[R],
                                   double (&vbot)[R], double (&dl)[C], double (&vl)[C]) {
     using PL = ChainPlan<R, C, G>;
     // One wait for the whole group (its loads were issued a group earlier),
@@ -265,7 +292,7 @@ __device__ inline void chain_rows(const double (&wv)[C][ChainPlan<R, C, G>::GMAX
 #pragma unroll
         for (int i = 0; i < PL::size(g); ++i)
 #pragma unroll
-            for (int k = 0; k < WK; ++k) asm volatile("" ::"v"(wv[c][i][k]));
+            for (int k = 0; k < mode_weights(MODE); ++k) asm volatile("" ::"v"(wv[c][i][k]));
 #pragma unroll
     for (int c = 0; c < C; ++c)
 #pragma unroll
@@ -278,7 +305,7 @@ __device__ inline void chain_rows(const double (&wv)[C][ChainPlan<R, C, G>::GMAX
             const double dn = dmin(dmax(d, dl[c]), dbot[t]);
             dbot[t] = dmax(d, dmin(dl[c], dbot[t]));
             dl[c] = dn;
-            if constexpr (WK == 3) {
+            if constexpr (mode_ve(MODE)) {
                 const double wr = wv[c][i][1], wt = wv[c][i][2];
                 const double vin = dmin(vl[c], vbot[t]);
                 vbot[t] = dmax(wt, vin);
@@ -302,9 +329,9 @@ __device__ inline void static_for(F &&f) {
 // at position (c + 63 C) mod KB, so that a step's positions follow its index
 // and a trip's never wrap.  Lanes 0-62 of the chain store their outputs to
 // slots of their own (odummy), so that the chain's store needs no lane mask.
-template <int R, int NPW, int C = 1, int WK = 3> struct BoundsShared {
-    static constexpr int KS = BoundsRing<R, NPW, C, WK>::KS;
-    static constexpr int KB = BoundsRing<R, NPW, C, WK>::KB;
+template <int R, int NPW, int C = 1, int MODE = kModeF64> struct BoundsShared {
+    static constexpr int KS = BoundsRing<R, NPW, C, MODE>::KS;
+    static constexpr int KB = BoundsRing<R, NPW, C, MODE>::KB;
     // The small arrays first, so that their addresses fit a DS instruction's
     // 16-bit offset.
     alignas(16) int sprod[16];  // steps each producer has written
@@ -316,8 +343,8 @@ template <int R, int NPW, int C = 1, int WK = 3> struct BoundsShared {
     alignas(16) double inring[2 * KB];
     alignas(16) double outring[2 * KB];
     alignas(16) double odummy[2 * (64 + C * KS)];
-    alignas(16) double qring[qring_comps(NPW, WK) * qring_cols(C)];
-    alignas(16) double wring[KS * C * R * WK * 64];
+    alignas(16) double qring[qring_comps(NPW, MODE) * qring_cols(C)];
+    alignas(16) double wring[KS * C * R * mode_weights(MODE) * 64];
 };
 
 // Steps of band `band`.  Lane l evaluates columns C (s - l) .. C (s - l) + C - 1
@@ -331,14 +358,14 @@ __device__ inline int band_steps(const BoundsArgs &a, const int band) {
     return (toff >= 0 && toff < 64LL * R) ? cols / C + (int)(toff / R) : cols / C + 63;
 }
 
-template <int R, int NPW, int C, int WK>
-__device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C, WK> &sh) {
+template <int R, int NPW, int C, int MODE>
+__device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C, MODE> &sh) {
     constexpr int GP = (NPW + 2 > 12) ? 3 : 0;  // four waves per SIMD: 128 VGPRs
     using PL = ChainPlan<R, C, GP>;
     constexpr int NG = PL::NG;
-    constexpr int KS = BoundsShared<R, NPW, C, WK>::KS;
-    constexpr int KB = BoundsShared<R, NPW, C, WK>::KB;
-    constexpr int SLOT = C * R * WK * 64;  // doubles per ring slot
+    constexpr int KS = BoundsShared<R, NPW, C, MODE>::KS;
+    constexpr int KB = BoundsShared<R, NPW, C, MODE>::KB;
+    constexpr int SLOT = C * R * mode_weights(MODE) * 64;  // doubles per ring slot
     constexpr int NONE = 0x3fffffff;      // no limit
     static_assert(NG >= 2, "the pipeline needs two row groups");
     const int lane = role_lane();
@@ -430,13 +457,13 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
     // weights occupy registers at a time.  Loads are issued on every path
     // (the last step reads a slot and ring positions nobody uses), so that
     // the compiler's LDS counter waits stay exact.
-    double w[NG][C][PL::GMAX][WK];
+    double w[NG][C][PL::GMAX][mode_weights(MODE)];
     double bD[C], bV[C];  // the next step's input from below (lane 0)
     auto load_in = [&](const double *p) __attribute__((always_inline)) {
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             bD[c] = p[2 * c];
-            if constexpr (WK == 3) bV[c] = p[2 * c + 1];
+            if constexpr (mode_ve(MODE)) bV[c] = p[2 * c + 1];
         }
     };
     wait_for(-1);
@@ -466,7 +493,7 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
             return wb[kk / NB] + (kk % NB) * SLOT;
         }
     };
-    chain_load<R, C, GP, 0, WK>(slot_ptr(std::integral_constant<int, 0>{}), w[0]);
+    chain_load<R, C, GP, 0, MODE>(slot_ptr(std::integral_constant<int, 0>{}), w[0]);
     load_in(sh.inring);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     // Steps finished, in a vector register: the store of the count then needs
@@ -489,12 +516,12 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             dl[c] = dpp_f64(bD[c], dout[c], 0);
-            if constexpr (WK == 3) vl[c] = dpp_f64(bV[c], vout[c], 0);
+            if constexpr (mode_ve(MODE)) vl[c] = dpp_f64(bV[c], vout[c], 0);
         }
         static_for<0, NG>([&](auto G) __attribute__((always_inline)) {
             constexpr int g = decltype(G)::value;
             if constexpr (g + 1 < NG) {
-                chain_load<R, C, GP, g + 1, WK>(wcur, w[g + 1]);
+                chain_load<R, C, GP, g + 1, MODE>(wcur, w[g + 1]);
                 if constexpr (g + 2 == NG) {
                     // The step's last loads from its slot are issued, and a
                     // wave's LDS operations complete in order: producers may
@@ -508,18 +535,18 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
                     lim = limit();
                     lrel = lim - s0;
                 }
-                chain_load<R, C, GP, 0, WK>(wnext, w[0]);
+                chain_load<R, C, GP, 0, MODE>(wnext, w[0]);
                 load_in(in_next);
             }
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            chain_rows<R, C, GP, g, WK>(w[g], dbot, vbot, dl, vl);
-            chain_done<R, C, GP, g, WK>(dbot, vbot, dl, vl);
+            chain_rows<R, C, GP, g, MODE>(w[g], dbot, vbot, dl, vl);
+            chain_done<R, C, GP, g, MODE>(dbot, vbot, dl, vl);
         });
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             dout[c] = dl[c];
             ow[2 * c] = dout[c];
-            if constexpr (WK == 3) {
+            if constexpr (mode_ve(MODE)) {
                 vout[c] = vl[c];
                 ow[2 * c + 1] = vout[c];
             }
@@ -566,9 +593,10 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
         }
         a.result[0] = __builtin_sqrt(topD);
         a.result[1] = topD;
-        if constexpr (WK == 3) {
+        if constexpr (mode_ve(MODE)) {
             const double resV = dmin(topV, vout[C - 1]);
-            a.result[2] = __builtin_sqrt(resV);
+            // The FP32 VE-FD's square root is FP32's (resV is an FP32 value).
+            a.result[2] = (MODE == kModeV32) ? (double)__builtin_sqrtf((float)resV) : __builtin_sqrt(resV);
             a.result[3] = resV;
         }
     }
@@ -594,9 +622,9 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
 // and its memory traffic falls from 5.9 to 2.5 GB.  Shorter steps publish
 // every column as soon as it is ready, since there the batches lengthen the
 // pipeline's fill (5% at 7 rows per lane on a 100k x 120k pair).
-template <int R, int NPW, int C, int WK>
-__device__ inline void bounds_courier(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C, WK> &sh) {
-    constexpr int KB = BoundsShared<R, NPW, C, WK>::KB;
+template <int R, int NPW, int C, int MODE>
+__device__ inline void bounds_courier(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C, MODE> &sh) {
+    constexpr int KB = BoundsShared<R, NPW, C, MODE>::KB;
     constexpr bool BATCH = R >= 13;
     constexpr int PUB = BATCH ? 16 : 1;
     const int lane = role_lane();
@@ -634,11 +662,11 @@ __device__ inline void bounds_courier(const BoundsArgs &a, const int band, Bound
                             vD = (c == 0) ? 0.0 : big;
                         } else {
                             vD = column_load<SYNC_SCOPED>(inD + c);
-                            if constexpr (WK == 3) vV = column_load<SYNC_SCOPED>(inV + c);
+                            if constexpr (mode_ve(MODE)) vV = column_load<SYNC_SCOPED>(inV + c);
                         }
                         const int p = c % KB;
                         sh.inring[2 * p] = vD;
-                        if constexpr (WK == 3) sh.inring[2 * p + 1] = vV;
+                        if constexpr (mode_ve(MODE)) sh.inring[2 * p + 1] = vV;
                     }
                     __atomic_signal_fence(__ATOMIC_SEQ_CST);
                     __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the ring writes are done
@@ -663,7 +691,7 @@ __device__ inline void bounds_courier(const BoundsArgs &a, const int band, Bound
                     const int c = out_next + lane;
                     const int p = (c + 63 * C) % KB;
                     column_store<SYNC_SCOPED>(outD + c, sh.outring[2 * p]);
-                    if constexpr (WK == 3) column_store<SYNC_SCOPED>(outV + c, sh.outring[2 * p + 1]);
+                    if constexpr (mode_ve(MODE)) column_store<SYNC_SCOPED>(outV + c, sh.outring[2 * p + 1]);
                 }
                 __builtin_amdgcn_wave_barrier();
                 if (lane == 0) flag_store<SYNC_SCOPED>(&a.progress[band], out_next + n);
@@ -697,7 +725,7 @@ template <int R, int NPW> struct ProducerRows {
 // `cons`: the chain's count of steps whose ring slot it has read.  Points are
 // DS doubles apart, of which the first D are read (the batch stores every
 // curve 3D: its 2D pairs run D = 2 over DS = 3, whose extra coordinate is 0).
-template <int D, int R, int NPW, int KS, int C, int NR, int DS = D, int WK = 3>
+template <int D, int R, int NPW, int KS, int C, int NR, int DS = D, int MODE = kModeF64>
 __device__ inline void bounds_producer(const BoundsArgs &a, const int band, const int pw, double *wring,
                                        double *qring, int *prod, int *cons) {
     constexpr int QR = qring_cols(C);
@@ -730,6 +758,23 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         ab2[k] = acc;
         lim2[k] = far_end_lim<D>(pm[k], ab[k], pi[k], acc);
     }
+    // The same terms of the points rounded to FP32, for the FP32 VE-FD.
+    float pif[RP][D], pmf[RP][D], abf[RP][D], ab2f[RP], lim2f[RP];
+    if constexpr (MODE == kModeV32) {
+#pragma unroll
+        for (int k = 0; k < RP; ++k) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                pif[k][c] = (float)pi[k][c];
+                pmf[k][c] = (k == 0) ? (float)pm[k][c] : pif[k - 1][c];
+                abf[k][c] = pif[k][c] - pmf[k][c];
+                acc = __builtin_fmaf(abf[k][c], abf[k][c], acc);
+            }
+            ab2f[k] = acc;
+            lim2f[k] = far_end_lim<D>(pmf[k], abf[k], pif[k], acc);
+        }
+    }
     // Q ring.  Lane l reads columns C(s - l) .. C(s - l) + C - 1 at step s.
     // Every producer wave stores each chunk itself (the same values): the
     // chunk after the one holding column Cs, whenever Cs is a multiple of 64,
@@ -738,13 +783,33 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
     // steps ahead of the chain), so no wave still reads the chunk 4C chunks
     // back, whose slots the stored chunk takes, and none reads a chunk before
     // its own store of it (a wave's LDS operations complete in order).
-    constexpr bool QT = qring_comps(NPW, WK) == 8;
+    constexpr bool QT = qring_comps(NPW, MODE) == 8;
+    constexpr bool QT32 = qring_comps(NPW, MODE) == 7;
+    static_assert(MODE != kModeV32 || QT32, "the FP32 VE-FD needs staged column terms (five or more producers)");
     // Stores column c's entries: q_c, and with the segment terms q_c -
     // q_{c-1}, its squared length and the weight test's limit (the operations
-    // of the column terms below, so the same values), q_{c-1} being `qm`.
+    // of the column terms below, so the same values), q_{c-1} being `qm`; for
+    // the FP32 VE-FD the same terms of q rounded to FP32, packed two FP32
+    // values to a double (q_c, the segment terms, length, limit).
     auto put_column = [&](const int idx, const double(&qc)[D], const double(&qm)[D]) __attribute__((always_inline)) {
 #pragma unroll
         for (int k = 0; k < D; ++k) qring[k * QR + idx] = qc[k];
+        if constexpr (QT32) {
+            float F[8] = {0, 0, 0, 0, 0, 0, 0, 0}, qcf[D], qmf[D], cabf[D], c2 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                qcf[k] = (float)qc[k];
+                qmf[k] = (float)qm[k];
+                cabf[k] = qcf[k] - qmf[k];
+                c2 = __builtin_fmaf(cabf[k], cabf[k], c2);
+                F[k] = qcf[k];
+                F[D + k] = cabf[k];
+            }
+            F[2 * D] = c2;
+            F[2 * D + 1] = far_end_lim<D>(qmf, cabf, qcf, c2);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) qring[(3 + u) * QR + idx] = pack_f2(F[2 * u], F[2 * u + 1]);
+        }
         if constexpr (QT) {
             double c2 = 0.0, cab[D];
 #pragma unroll
@@ -783,24 +848,37 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
     // and that DFD value |p_i - q_{j-1}|^2, the right weight at t = 0.
     // Values at j <= 0 feed only weights that the boundary rules below
     // overwrite.
-    double qA[D], qB[D], xA[RP][D], xB[RP][D], dA[RP], dB[RP];
+    // (For the FP32 VE-FD the same differences and values in FP32.)
+    struct Carry {
+        double q[D], x[RP][D], d[RP];
+        float xf[RP][D], df[RP];
+    };
+    Carry cA, cB;
 #pragma unroll
-    for (int c = 0; c < D; ++c) qA[c] = 0.0;
+    for (int c = 0; c < D; ++c) cA.q[c] = 0.0;
 #pragma unroll
     for (int k = 0; k < RP; ++k) {
-        dA[k] = 0.0;
+        cA.d[k] = 0.0;
+        cA.df[k] = 0.0f;
 #pragma unroll
-        for (int c = 0; c < D; ++c) xA[k][c] = 0.0;
+        for (int c = 0; c < D; ++c) {
+            cA.x[k][c] = 0.0;
+            cA.xf[k][c] = 0.0f;
+        }
     }
 
     // q of the wave's next column, loaded a column ahead (a wave's columns
     // follow each other: j, j + 1, ...).  At j < 0 it reads ring slots not yet
     // holding Q; those values feed only weights that the boundary rules
     // overwrite.
-    double qp[D], cp[D], c2p = 0.0, lp = 0.0;
+    double qp[D], cp[D], c2p = 0.0, lp = 0.0, fp[4];
     auto prefetch = [&](const int qi) __attribute__((always_inline)) {
 #pragma unroll
         for (int k = 0; k < D; ++k) qp[k] = qring[k * QR + qi];
+        if constexpr (QT32) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) fp[u] = qring[(3 + u) * QR + qi];
+        }
         if constexpr (QT) {
 #pragma unroll
             for (int k = 0; k < D; ++k) cp[k] = qring[(3 + k) * QR + qi];
@@ -811,10 +889,15 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
     prefetch((C * (0 - lane)) & (QR - 1));
 
     // Weights of column j for this wave's rows into w (one column of a slot).
-    auto column = [&](const int j, const double(&qa)[D], double(&qb)[D], const double(&xp)[RP][D],
-                      const double(&dp)[RP], double(&xn)[RP][D], double(&dn)[RP], double *w, const bool edge)
+    auto column = [&](const int j, const Carry &cp_, Carry &cn_, double *w, const bool edge)
                       __attribute__((always_inline)) {
-        if constexpr (WK == 1) {
+        const double(&qa)[D] = cp_.q;
+        double(&qb)[D] = cn_.q;
+        const double(&xp)[RP][D] = cp_.x;
+        const double(&dp)[RP] = cp_.d;
+        double(&xn)[RP][D] = cn_.x;
+        double(&dn)[RP] = cn_.d;
+        if constexpr (MODE == kModeDfd) {
             // The DFD-only sweep: each cell's distance^2, and +big before a
             // lane's first column (the row's initial value).
 #pragma unroll
@@ -832,88 +915,174 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
             }
             return;
         }
-        // Column terms of segment [q_{j-1}, q_j].
-        double cab[D], cab2 = 0.0, clim;
+        if constexpr (MODE == kModeV32) {
+            // DFD in FP64; VE-FD weights in FP32 from the points rounded to
+            // FP32, by the same rules, in FP32's operations.
 #pragma unroll
-        for (int k = 0; k < D; ++k) qb[k] = qp[k];
-        if constexpr (QT) {
+            for (int k = 0; k < D; ++k) qb[k] = qp[k];
+            float F[8];
 #pragma unroll
-            for (int k = 0; k < D; ++k) cab[k] = cp[k];
-            cab2 = c2p;
-            clim = lp;
-        } else {
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-                cab[k] = qb[k] - qa[k];
-                cab2 = __builtin_fma(cab[k], cab[k], cab2);
+            for (int u = 0; u < 4; ++u) {
+                F[2 * u] = unpack_lo(fp[u]);
+                F[2 * u + 1] = unpack_hi(fp[u]);
             }
-            clim = far_end_lim<D>(qa, cab, qb, cab2);
-        }
-        prefetch((j + 1) & (QR - 1));
-        // Per row: the projections' dot products, the weights at a clamped
-        // parameter (seg_t_slow's rule) and whether a weight needs computing.
-        double p1[RP], p2[RP], dt[RP][D], wr[RP], wt[RP];
-        bool s1[RP], s2[RP], slow = false;
-#pragma unroll
-        for (int k = 0; k < RP; ++k) {
-            double d = 0.0;
+            float qf[D], cabf[D];
 #pragma unroll
             for (int c = 0; c < D; ++c) {
-                xn[k][c] = pi[k][c] - qb[c];
-                d = __builtin_fma(xn[k][c], xn[k][c], d);
+                qf[c] = F[c];
+                cabf[c] = F[D + c];
             }
-            // The top weight's q_j - p_{i-1}: for a row after the wave's first,
-            // -(p_{i-1} - q_j), the row before's difference (round to nearest
-            // is symmetric; a zero's sign cannot change the projection's sum,
-            // which starts at +0).
-#pragma unroll
-            for (int c = 0; c < D; ++c) dt[k][c] = (k == 0) ? qb[c] - pm[k][c] : -xn[k - 1][c];
-            // |q_j - p_{i-1}|^2, the top weight at t = 0: for a row after the
-            // wave's first, the row before's DFD value (its terms are dt's
-            // negations, whose squares are the same).
-            double nt = 0.0;
-            if (k == 0) {
-#pragma unroll
-                for (int c = 0; c < D; ++c) nt = __builtin_fma(dt[k][c], dt[k][c], nt);
-            } else {
-                nt = dn[k - 1];
-            }
-            // Right weight: p_i to [q_{j-1}, q_j] (ap = p_i - q_{j-1}); top
-            // weight: q_j to [p_{i-1}, p_i] (ap = q_j - p_{i-1}).
-            double a1 = 0.0, a2 = 0.0;
-#pragma unroll
-            for (int c = 0; c < D; ++c) a1 = __builtin_fma(xp[k][c], cab[c], a1);
-#pragma unroll
-            for (int c = 0; c < D; ++c) a2 = __builtin_fma(dt[k][c], ab[k][c], a2);
-            p1[k] = a1;
-            p2[k] = a2;
-            wr[k] = a1 > 0.0 ? d : dp[k];
-            wt[k] = a2 > 0.0 ? d : nt;
-            s1[k] = a1 > 0.0 && a1 < clim;
-            s2[k] = a2 > 0.0 && a2 < lim2[k];
-            slow = slow || s1[k] || s2[k];
-            dn[k] = d;
-        }
-        if (slow) {
-            asm volatile("");  // one branch for the wave's rows, not selects
+            const float c2f = F[2 * D], climf = F[2 * D + 1];
+            prefetch((j + 1) & (QR - 1));
+            float a1f[RP], a2f[RP], wrf[RP], wtf[RP];
+            bool s1[RP], s2[RP], slow = false;
 #pragma unroll
             for (int k = 0; k < RP; ++k) {
-                if (s1[k]) {
-                    asm volatile("");
-                    wr[k] = seg_w_at<D>(pi[k], qa, cab, seg_t_slow(p1[k], cab2));
+                double d = 0.0;
+#pragma unroll
+                for (int c = 0; c < D; ++c) {
+                    const double x = pi[k][c] - qb[c];
+                    d = __builtin_fma(x, x, d);
                 }
-                if (s2[k]) {
-                    asm volatile("");
-                    wt[k] = seg_w_at<D>(qb, pm[k], ab[k], seg_t_slow(p2[k], ab2[k]));
+                dn[k] = d;
+                float e = 0.0f;
+#pragma unroll
+                for (int c = 0; c < D; ++c) {
+                    cn_.xf[k][c] = pif[k][c] - qf[c];
+                    e = __builtin_fmaf(cn_.xf[k][c], cn_.xf[k][c], e);
+                }
+                float dtf[D];
+#pragma unroll
+                for (int c = 0; c < D; ++c) dtf[c] = (k == 0) ? qf[c] - pmf[k][c] : -cn_.xf[k - 1][c];
+                float nt = 0.0f;
+                if (k == 0) {
+#pragma unroll
+                    for (int c = 0; c < D; ++c) nt = __builtin_fmaf(dtf[c], dtf[c], nt);
+                } else {
+                    nt = cn_.df[k - 1];
+                }
+                float x1 = 0.0f, x2 = 0.0f;
+#pragma unroll
+                for (int c = 0; c < D; ++c) x1 = __builtin_fmaf(cp_.xf[k][c], cabf[c], x1);
+#pragma unroll
+                for (int c = 0; c < D; ++c) x2 = __builtin_fmaf(dtf[c], abf[k][c], x2);
+                a1f[k] = x1;
+                a2f[k] = x2;
+                wrf[k] = x1 > 0.0f ? e : cp_.df[k];
+                wtf[k] = x2 > 0.0f ? e : nt;
+                s1[k] = x1 > 0.0f && x1 < climf;
+                s2[k] = x2 > 0.0f && x2 < lim2f[k];
+                slow = slow || s1[k] || s2[k];
+                cn_.df[k] = e;
+            }
+            if (slow) {
+                asm volatile("");  // one branch for the wave's rows, not selects
+                float qmf[D];  // q_{j-1} rounded to FP32
+#pragma unroll
+                for (int c = 0; c < D; ++c) qmf[c] = (float)qa[c];
+#pragma unroll
+                for (int k = 0; k < RP; ++k) {
+                    if (s1[k]) {
+                        asm volatile("");
+                        wrf[k] = seg_w_at<D>(pif[k], qmf, cabf, seg_t_slow(a1f[k], c2f));
+                    }
+                    if (s2[k]) {
+                        asm volatile("");
+                        wtf[k] = seg_w_at<D>(qf, pmf[k], abf[k], seg_t_slow(a2f[k], ab2f[k]));
+                    }
                 }
             }
-        }
 #pragma unroll
-        for (int k = 0; k < RP; ++k) {
-            const int t = tfirst + k;
-            w[(t * 3 + 0) * 64] = dn[k];
-            w[(t * 3 + 1) * 64] = wr[k];
-            w[(t * 3 + 2) * 64] = wt[k];
+            for (int k = 0; k < RP; ++k) {
+                const int t = tfirst + k;
+                w[(t * 3 + 0) * 64] = dn[k];
+                w[(t * 3 + 1) * 64] = (double)wrf[k];
+                w[(t * 3 + 2) * 64] = (double)wtf[k];
+            }
+        } else {
+            // Column terms of segment [q_{j-1}, q_j].
+            double cab[D], cab2 = 0.0, clim;
+#pragma unroll
+            for (int k = 0; k < D; ++k) qb[k] = qp[k];
+            if constexpr (QT) {
+#pragma unroll
+                for (int k = 0; k < D; ++k) cab[k] = cp[k];
+                cab2 = c2p;
+                clim = lp;
+            } else {
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    cab[k] = qb[k] - qa[k];
+                    cab2 = __builtin_fma(cab[k], cab[k], cab2);
+                }
+                clim = far_end_lim<D>(qa, cab, qb, cab2);
+            }
+            prefetch((j + 1) & (QR - 1));
+            // Per row: the projections' dot products, the weights at a clamped
+            // parameter (seg_t_slow's rule) and whether a weight needs computing.
+            double p1[RP], p2[RP], dt[RP][D], wr[RP], wt[RP];
+            bool s1[RP], s2[RP], slow = false;
+#pragma unroll
+            for (int k = 0; k < RP; ++k) {
+                double d = 0.0;
+#pragma unroll
+                for (int c = 0; c < D; ++c) {
+                    xn[k][c] = pi[k][c] - qb[c];
+                    d = __builtin_fma(xn[k][c], xn[k][c], d);
+                }
+                // The top weight's q_j - p_{i-1}: for a row after the wave's first,
+                // -(p_{i-1} - q_j), the row before's difference (round to nearest
+                // is symmetric; a zero's sign cannot change the projection's sum,
+                // which starts at +0).
+#pragma unroll
+                for (int c = 0; c < D; ++c) dt[k][c] = (k == 0) ? qb[c] - pm[k][c] : -xn[k - 1][c];
+                // |q_j - p_{i-1}|^2, the top weight at t = 0: for a row after the
+                // wave's first, the row before's DFD value (its terms are dt's
+                // negations, whose squares are the same).
+                double nt = 0.0;
+                if (k == 0) {
+#pragma unroll
+                    for (int c = 0; c < D; ++c) nt = __builtin_fma(dt[k][c], dt[k][c], nt);
+                } else {
+                    nt = dn[k - 1];
+                }
+                // Right weight: p_i to [q_{j-1}, q_j] (ap = p_i - q_{j-1}); top
+                // weight: q_j to [p_{i-1}, p_i] (ap = q_j - p_{i-1}).
+                double a1 = 0.0, a2 = 0.0;
+#pragma unroll
+                for (int c = 0; c < D; ++c) a1 = __builtin_fma(xp[k][c], cab[c], a1);
+#pragma unroll
+                for (int c = 0; c < D; ++c) a2 = __builtin_fma(dt[k][c], ab[k][c], a2);
+                p1[k] = a1;
+                p2[k] = a2;
+                wr[k] = a1 > 0.0 ? d : dp[k];
+                wt[k] = a2 > 0.0 ? d : nt;
+                s1[k] = a1 > 0.0 && a1 < clim;
+                s2[k] = a2 > 0.0 && a2 < lim2[k];
+                slow = slow || s1[k] || s2[k];
+                dn[k] = d;
+            }
+            if (slow) {
+                asm volatile("");  // one branch for the wave's rows, not selects
+#pragma unroll
+                for (int k = 0; k < RP; ++k) {
+                    if (s1[k]) {
+                        asm volatile("");
+                        wr[k] = seg_w_at<D>(pi[k], qa, cab, seg_t_slow(p1[k], cab2));
+                    }
+                    if (s2[k]) {
+                        asm volatile("");
+                        wt[k] = seg_w_at<D>(qb, pm[k], ab[k], seg_t_slow(p2[k], ab2[k]));
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < RP; ++k) {
+                const int t = tfirst + k;
+                w[(t * 3 + 0) * 64] = dn[k];
+                w[(t * 3 + 1) * 64] = wr[k];
+                w[(t * 3 + 2) * 64] = wt[k];
+            }
         }
         if (edge) {
             // Boundary and padding cells overwrite their VE-FD weights.  A real
@@ -950,7 +1119,7 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
             // q_{c-1}: lane l-1's q, and for lane 0 the chunk before's last
             // column, already in the ring.
             double qm[D];
-            if constexpr (QT) {
+            if constexpr (QT || QT32) {
 #pragma unroll
                 for (int k = 0; k < D; ++k) qm[k] = dpp_f64(qring[k * QR + ((C * s + 63) & (QR - 1))], qn[k], 0);
             }
@@ -985,11 +1154,11 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         // one column to the next instead of being copied.
         for (int s = 0; s < nsteps; ++s) {
             step_begin(s, 0);
-            double *w = wring + slot * (C * R * WK * 64) + lane;
+            double *w = wring + slot * (C * R * mode_weights(MODE) * 64) + lane;
             const bool edge = s < 64 || edge_band;
             const int j = C * (s - lane);
-            column(j, qA, qB, xA, dA, xB, dB, w, edge);
-            column(j + 1, qB, qA, xB, dB, xA, dA, w + R * WK * 64, edge);
+            column(j, cA, cB, w, edge);
+            column(j + 1, cB, cA, w + R * mode_weights(MODE) * 64, edge);
             step_end(s);
         }
     } else {
@@ -1013,12 +1182,11 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         // their first column) and in the first and last bands (row 0, padding
         // rows): those steps run a copy of the loop that checks them, the
         // others one without the check.
-        auto one = [&](auto edge_c, const int s, const double(&qa)[D], double(&qb)[D], const double(&xp)[RP][D],
-                       const double(&dp)[RP], double(&xn)[RP][D], double(&dn)[RP], const bool first, const bool last,
+        auto one = [&](auto edge_c, const int s, const Carry &cp_, Carry &cn_, const bool first, const bool last,
                        const int ahead) __attribute__((always_inline)) {
             if (first) step_begin(s, last ? 0 : ahead);
-            double *w = wring + slot * (R * WK * 64) + lane;
-            column(s - lane, qa, qb, xp, dp, xn, dn, w, decltype(edge_c)::value);
+            double *w = wring + slot * (R * mode_weights(MODE) * 64) + lane;
+            column(s - lane, cp_, cn_, w, decltype(edge_c)::value);
             if (last) {
                 step_end(s);
             } else {
@@ -1029,18 +1197,18 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         auto steps_to = [&](auto e, const int end) __attribute__((always_inline)) {
             if constexpr (QUADS) {
                 for (; s + 3 < end; s += 4) {
-                    one(e, s, qA, qB, xA, dA, xB, dB, true, false, 3);
-                    one(e, s + 1, qB, qA, xB, dB, xA, dA, false, false, 0);
-                    one(e, s + 2, qA, qB, xA, dA, xB, dB, false, false, 0);
-                    one(e, s + 3, qB, qA, xB, dB, xA, dA, false, true, 0);
+                    one(e, s, cA, cB, true, false, 3);
+                    one(e, s + 1, cB, cA, false, false, 0);
+                    one(e, s + 2, cA, cB, false, false, 0);
+                    one(e, s + 3, cB, cA, false, true, 0);
                 }
             }
             for (; s + 1 < end; s += 2) {
-                one(e, s, qA, qB, xA, dA, xB, dB, true, false, 1);
-                one(e, s + 1, qB, qA, xB, dB, xA, dA, false, true, 0);
+                one(e, s, cA, cB, true, false, 1);
+                one(e, s + 1, cB, cA, false, true, 0);
             }
             if (s < end) {
-                one(e, s, qA, qB, xA, dA, xB, dB, true, true, 0);
+                one(e, s, cA, cB, true, true, 0);
                 ++s;
             }
         };
@@ -1053,11 +1221,11 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
 
 // One band of one pair, by every wave of the workgroup.  The caller has
 // cleared the sync words and ends with a barrier before reusing the LDS.
-// C = 2 (two columns per step) requires n even.  DS: the points' stride.  WK:
-// 3 for DFD and VE-FD, 1 for DFD alone.
-template <int D, int R, int NPW, int C = 1, int DS = D, int WK = 3>
-__device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C, WK> &sh) {
-    constexpr int KS = BoundsRing<R, NPW, C, WK>::KS;
+// C = 2 (two columns per step) requires n even.  DS: the points' stride.  MODE:
+// the sweep mode (kModeF64, kModeDfd, kModeV32).
+template <int D, int R, int NPW, int C = 1, int DS = D, int MODE = kModeF64>
+__device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsShared<R, NPW, C, MODE> &sh) {
+    constexpr int KS = BoundsRing<R, NPW, C, MODE>::KS;
     // The wave index as a value the compiler cannot hoist out of the band
     // loop, so that no role's set-up is computed at kernel entry and held
     // (spilled) across the other roles' loops.
@@ -1069,22 +1237,22 @@ __device__ inline void bounds_band(const BoundsArgs &a, const int band, BoundsSh
     constexpr int cw = (NPW + 2 > 4) ? 4 : NPW + 1;
     constexpr int RP = (R + NPW - 1) / NPW;
     if (wave == 0) {
-        bounds_chain<R, NPW, C, WK>(a, band, sh);
+        bounds_chain<R, NPW, C, MODE>(a, band, sh);
     } else if (wave == cw) {
-        bounds_courier<R, NPW, C, WK>(a, band, sh);
+        bounds_courier<R, NPW, C, MODE>(a, band, sh);
     } else {
         const int pw = wave - 1 - (wave > cw ? 1 : 0);
         if (RP * NPW == R || pw < R - (RP - 1) * NPW) {
-            bounds_producer<D, R, NPW, KS, C, RP, DS, WK>(a, band, pw, sh.wring, sh.qring, sh.sprod, &sh.wfree[0]);
+            bounds_producer<D, R, NPW, KS, C, RP, DS, MODE>(a, band, pw, sh.wring, sh.qring, sh.sprod, &sh.wfree[0]);
         } else if constexpr (RP > 1) {
-            bounds_producer<D, R, NPW, KS, C, RP - 1, DS, WK>(a, band, pw, sh.wring, sh.qring, sh.sprod,
+            bounds_producer<D, R, NPW, KS, C, RP - 1, DS, MODE>(a, band, pw, sh.wring, sh.qring, sh.sprod,
                                                                 &sh.wfree[0]);
         }
     }
 }
 
-template <int R, int NPW, int C, int WK>
-__device__ inline void bounds_clear(BoundsShared<R, NPW, C, WK> &sh) {
+template <int R, int NPW, int C, int MODE>
+__device__ inline void bounds_clear(BoundsShared<R, NPW, C, MODE> &sh) {
     for (int w = 0; w < NPW; ++w) sh.sprod[w] = 0;
     sh.cons[0] = 0;
     sh.wfree[0] = 0;
@@ -1092,13 +1260,14 @@ __device__ inline void bounds_clear(BoundsShared<R, NPW, C, WK> &sh) {
     sh.out_done = 0;
 }
 
-// WK = 3: DFD and VE-FD (result[0..3]); WK = 1: DFD alone (result[0..1]).
-template <int D, int R, int NPW, int C, int WK = 3>
+// MODE: kModeF64 and kModeV32 give DFD and VE-FD (result[0..3]), kModeDfd DFD
+// alone (result[0..1]).
+template <int D, int R, int NPW, int C, int MODE = kModeF64>
 __global__ __launch_bounds__(64 * (NPW + 2)) void bounds_sweep(BoundsArgs a) {
     static_assert(NPW >= 1 && NPW <= R, "one to R producer waves");
     static_assert(C == 1 || C == 2, "one or two columns per step");
-    static_assert(WK == 1 || WK == 3, "DFD alone, or DFD and VE-FD");
-    __shared__ BoundsShared<R, NPW, C, WK> sh;
+    static_assert(MODE == kModeF64 || MODE == kModeDfd || MODE == kModeV32, "a sweep mode");
+    __shared__ BoundsShared<R, NPW, C, MODE> sh;
     for (;;) {
         if (threadIdx.x == 0) {
             sh.item = atomicAdd(a.queue, 1);
@@ -1107,7 +1276,7 @@ __global__ __launch_bounds__(64 * (NPW + 2)) void bounds_sweep(BoundsArgs a) {
         __syncthreads();
         const int band = sh.item;
         if (band >= a.nbands) break;
-        bounds_band<D, R, NPW, C, D, WK>(a, band, sh);
+        bounds_band<D, R, NPW, C, D, MODE>(a, band, sh);
         __syncthreads();
     }
 }

@@ -21,9 +21,9 @@ namespace {
 // Configurations: rows per lane R and producer waves NPW (one chain wave and
 // one courier wave besides), encoded as 10 * R + NPW (100 * R + NPW from 10
 // producer waves on), plus 20000 for two columns per step.
-template <int D, int R, int NPW, int C = 1, int WK = 3>
+template <int D, int R, int NPW, int C = 1, int MODE = rfd::kModeF64>
 hipError_t launch_bounds_cfg(rfd_ctx *ctx, int64_t nb, const rfd::BoundsArgs &a, int *grid_out) {
-    auto kern = rfd::bounds_sweep<D, R, NPW, C, WK>;
+    auto kern = rfd::bounds_sweep<D, R, NPW, C, MODE>;
     // Optional cap on workgroups per CU (RFD_WGCU): unused dynamic LDS sized so
     // that one more workgroup does not fit.  Bands on a CU shared by more
     // workgroups run slower, and every later band waits on the slowest.
@@ -74,10 +74,22 @@ hipError_t launch_bounds_d(rfd_ctx *ctx, int cfg, int64_t nb, const rfd::BoundsA
 template <int D>
 hipError_t launch_dfd_d(rfd_ctx *ctx, int cfg, int64_t nb, const rfd::BoundsArgs &a, int *grid) {
     switch (cfg) {
-        case 42: return launch_bounds_cfg<D, 4, 2, 1, 1>(ctx, nb, a, grid);
-        case 77: return launch_bounds_cfg<D, 7, 7, 1, 1>(ctx, nb, a, grid);
-        case 107: return launch_bounds_cfg<D, 10, 7, 1, 1>(ctx, nb, a, grid);
-        case 1310: return launch_bounds_cfg<D, 13, 10, 1, 1>(ctx, nb, a, grid);
+        case 42: return launch_bounds_cfg<D, 4, 2, 1, rfd::kModeDfd>(ctx, nb, a, grid);
+        case 77: return launch_bounds_cfg<D, 7, 7, 1, rfd::kModeDfd>(ctx, nb, a, grid);
+        case 107: return launch_bounds_cfg<D, 10, 7, 1, rfd::kModeDfd>(ctx, nb, a, grid);
+        case 1310: return launch_bounds_cfg<D, 13, 10, 1, rfd::kModeDfd>(ctx, nb, a, grid);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// DFD in FP64 and VE-FD in FP32, in the configurations with staged column
+// terms.
+template <int D>
+hipError_t launch_mixed_d(rfd_ctx *ctx, int cfg, int64_t nb, const rfd::BoundsArgs &a, int *grid) {
+    switch (cfg) {
+        case 77: return launch_bounds_cfg<D, 7, 7, 1, rfd::kModeV32>(ctx, nb, a, grid);
+        case 107: return launch_bounds_cfg<D, 10, 7, 1, rfd::kModeV32>(ctx, nb, a, grid);
+        case 1310: return launch_bounds_cfg<D, 13, 10, 1, rfd::kModeV32>(ctx, nb, a, grid);
         default: return hipErrorInvalidValue;
     }
 }
@@ -107,6 +119,12 @@ hipError_t rfd_launch_bounds(rfd_ctx *ctx, int dims, int cfg, int64_t nb, const 
 }
 
 bool rfd_valid_dfd_cfg(int cfg) { return cfg == 42 || cfg == 77 || cfg == 107 || cfg == 1310; }
+
+bool rfd_valid_mixed_cfg(int cfg) { return cfg == 77 || cfg == 107 || cfg == 1310; }
+
+hipError_t rfd_launch_mixed(rfd_ctx *ctx, int dims, int cfg, int64_t nb, const rfd::BoundsArgs &a, int *grid) {
+    return dims == 2 ? launch_mixed_d<2>(ctx, cfg, nb, a, grid) : launch_mixed_d<3>(ctx, cfg, nb, a, grid);
+}
 
 hipError_t rfd_launch_dfd(rfd_ctx *ctx, int dims, int cfg, int64_t nb, const rfd::BoundsArgs &a, int *grid) {
     return dims == 2 ? launch_dfd_d<2>(ctx, cfg, nb, a, grid) : launch_dfd_d<3>(ctx, cfg, nb, a, grid);

@@ -138,6 +138,8 @@ using rfd_host::endpoint_bound;
 using rfd_host::rfd_launch_bounds;
 using rfd_host::rfd_launch_dfd;
 using rfd_host::rfd_valid_dfd_cfg;
+using rfd_host::rfd_launch_mixed;
+using rfd_host::rfd_valid_mixed_cfg;
 using rfd_host::rfd_valid_batch_bounds_cfg;
 using rfd_host::rfd_valid_bounds_cfg;
 
@@ -319,11 +321,13 @@ rfd_status run_sweep(rfd_session *s, const T *P, const T *Q, const T *qseg, doub
     return RFD_OK;
 }
 
-// One fused bound sweep of a session (m, n >= 2): DFD, DFD^2 and VE-FD; with
-// `ve` false its DFD-only form (DFD and DFD^2; *ran false when the
-// configuration has no DFD-only form, and the caller runs band_sweep).
-rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, double *vefd, bool ve = true,
-                      bool *ran = nullptr) {
+// One fused bound sweep of a session (m, n >= 2): DFD, DFD^2 and VE-FD
+// (mode kModeF64), VE-FD in FP32 (kModeV32; `init` is then rounded to FP32
+// and squared in FP32), or DFD and DFD^2 alone (kModeDfd).  *ran is false
+// when the configuration has no such form, and the caller runs band_sweep.
+rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, double *vefd,
+                      int mode = rfd::kModeF64, bool *ran = nullptr) {
+    const bool ve = rfd::mode_ve(mode);
     rfd_ctx *ctx = s->ctx;
     // Configuration: RFD_BOUNDS if set; otherwise one band per CU with the
     // fewest rows per lane (7, 10 or 13) that cover the pair (7 producer
@@ -344,7 +348,7 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
     }
     if (cfg >= 20000 && (s->n & 1)) cfg -= 20000;
     if (ran) *ran = true;
-    if (!ve && !rfd_valid_dfd_cfg(cfg)) {
+    if ((mode == rfd::kModeDfd && !rfd_valid_dfd_cfg(cfg)) || (mode == rfd::kModeV32 && !rfd_valid_mixed_cfg(cfg))) {
         if (ran) *ran = false;
         return RFD_OK;
     }
@@ -362,7 +366,7 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
     a.rows = rows;
     a.cols = cols;
     a.nbands = (int)nb;
-    a.r2 = init * init;
+    a.r2 = (mode == rfd::kModeV32) ? (double)((float)init * (float)init) : init * init;
     a.watch = rfd::Watch{s->abort, ctx->spin_limit};
     const double sentinel = -3.0;
     unsigned long long *tlog = nullptr;
@@ -377,8 +381,10 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
                        sentinel, s->evals, s->abort);
     RFD_HIP(ctx, hipGetLastError());
     int grid = 0;
-    if (ve) {
+    if (mode == rfd::kModeF64) {
         RFD_HIP(ctx, rfd_launch_bounds(ctx, s->dims, cfg, nb, a, &grid));
+    } else if (mode == rfd::kModeV32) {
+        RFD_HIP(ctx, rfd_launch_mixed(ctx, s->dims, cfg, nb, a, &grid));
     } else {
         RFD_HIP(ctx, rfd_launch_dfd(ctx, s->dims, cfg, nb, a, &grid));
     }
@@ -393,7 +399,7 @@ rfd_status run_bounds(rfd_session *s, double init, double *dfd, double *dfd_sq, 
     // Cells of both bound sweeps: m * n DFD cells and (m-1)(n-1) VE-FD cells;
     // the DFD-only form counts as a DFD sweep.
     const double cells = (double)rows * (double)cols + (ve ? (double)(rows - 1) * (double)(cols - 1) : 0.0);
-    const int kind = ve ? RFD_K_BOUNDS64 : RFD_K_DFD;
+    const int kind = mode == rfd::kModeF64 ? RFD_K_BOUNDS64 : mode == rfd::kModeV32 ? RFD_K_BOUNDS_MIXED : RFD_K_DFD;
     ctx->stats.launches[kind] += 1;
     ctx->stats.ms[kind] += ms;
     ctx->stats.cells[kind] += cells;
@@ -566,7 +572,7 @@ rfd_status refined_ub(rfd_session *s, int levels, double *ub) {
         // The fused sweep's DFD-only form where it has one, else the band sweep.
         double F = 0;
         bool ran = false;
-        if (ctx->fused && t.m >= 2 && t.n >= 2) st = run_bounds(&t, 0.0, ub, &F, nullptr, false, &ran);
+        if (ctx->fused && t.m >= 2 && t.n >= 2) st = run_bounds(&t, 0.0, ub, &F, nullptr, rfd::kModeDfd, &ran);
         if (st == RFD_OK && !ran)
             st = run_sweep<rfd::OP_DFD, double>(&t, t.dP, t.dQ, nullptr, 0.0, RFD_K_DFD, ub, INFINITY, &F);
     } else if (st == RFD_ERR_NOMEM) {
@@ -820,11 +826,20 @@ rfd_status rfd_session_bounds(rfd_session *s, double *dfd, double *vefd, int run
         if (st == RFD_OK) s->dfd_sq = F;
         return st;
     }
-    // DFD alone: the fused sweep's DFD-only form where it has one, else the
-    // band sweep.
     bool ran = false;
     rfd_status st = RFD_OK;
-    if (ctx->fused && s->m >= 2 && s->n >= 2) st = run_bounds(s, 0.0, dfd, &F, nullptr, false, &ran);
+    // DFD in FP64 and VE-FD in FP32 by one fused sweep, where the
+    // configuration has that form.
+    if (ctx->fused && run_vefd && fp32 && !ctx->prune && s->m >= 2 && s->n >= 2) {
+        st = run_bounds(s, *vefd, dfd, &F, vefd, rfd::kModeV32, &ran);
+        if (st != RFD_OK || ran) {
+            if (st == RFD_OK) s->dfd_sq = F;
+            return st;
+        }
+    }
+    // DFD alone: the fused sweep's DFD-only form where it has one, else the
+    // band sweep.
+    if (ctx->fused && s->m >= 2 && s->n >= 2) st = run_bounds(s, 0.0, dfd, &F, nullptr, rfd::kModeDfd, &ran);
     if (st != RFD_OK) return st;
     if (!ran) st = run_sweep<rfd::OP_DFD, double>(s, s->dP, s->dQ, nullptr, 0.0, RFD_K_DFD, dfd, INFINITY, &F);
     if (st != RFD_OK) return st;

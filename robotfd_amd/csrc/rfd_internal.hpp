@@ -67,6 +67,9 @@ hipError_t rfd_launch_bounds(rfd_ctx *ctx, int dims, int cfg, int64_t nb, const 
 // The fused sweep's DFD-only form (configurations 42, 77, 107, 1310).
 bool rfd_valid_dfd_cfg(int cfg);
 hipError_t rfd_launch_dfd(rfd_ctx *ctx, int dims, int cfg, int64_t nb, const rfd::BoundsArgs &a, int *grid);
+// The fused sweep with VE-FD in FP32 (configurations 77, 107, 1310).
+bool rfd_valid_mixed_cfg(int cfg);
+hipError_t rfd_launch_mixed(rfd_ctx *ctx, int dims, int cfg, int64_t nb, const rfd::BoundsArgs &a, int *grid);
 bool rfd_valid_batch_bounds_cfg(int cfg);
 void rfd_launch_batch_bounds(rfd_ctx *ctx, int cfg, hipStream_t st, const rfd::BatchArgs &a);
 

@@ -62,10 +62,12 @@ def check_pair(ctx, P, Q):
 
 
 # The DFD-only form (one weight per cell, no VE-FD), which computes the DFD of
-# the FP32 and pruned bound paths, in every configuration it is built in.
+# the pruned bound path and of the FP32 one in configurations without the
+# mixed form, and the mixed form (DFD in FP64, VE-FD in FP32: configurations
+# 77, 107 and 1310), in every configuration they are built in.
 @pytest.mark.parametrize("cfg", [42, 77, 107, 1310])
 @pytest.mark.parametrize("dims", [2, 3])
-def test_dfd_only_configs_bit_exact(contexts, cfg, dims):
+def test_dfd_only_and_mixed_configs_bit_exact(contexts, cfg, dims):
     ctx = contexts(RFD_BOUNDS=cfg)
     for m, n in SHAPES:
         rng = np.random.default_rng(m * 1000 + n + dims + cfg + 7)
@@ -75,9 +77,12 @@ def test_dfd_only_configs_bit_exact(contexts, cfg, dims):
         with Intersection(P, Q, ctx=ctx) as s:
             dfd, _ = s.Bound_call_gpu(lb, run_vefd=False)
             assert dfd == O.dfd(P, Q)
-            # The FP32 bound path: DFD by the same form, VE-FD in FP32.
+            # The FP32 bound path: DFD in FP64, VE-FD in FP32.
+            ctx.reset_stats()
             dfd32, ve32 = s.Bound_call_gpu(lb, fp32=True)
             assert dfd32 == dfd and ve32 == O.vefd(P, Q, lb, fp32=True)
+            if m >= 2 and n >= 2 and cfg != 42:
+                assert ctx.stats()["bounds_mixed"]["launches"] == 1
 
 
 # 20000 + c: the same configuration at two columns per step (even n; odd n
