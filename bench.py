@@ -349,7 +349,9 @@ def profile_mode(what: str, points: int) -> int:
     P, Q = W.long_3d(points, points, seed=0)
     ctx = Context.default()
     with Intersection(P, Q, ctx=ctx) as s:
-        dfd, ve = s.Bound_call_gpu(0.0)
+        # "mixed": the bound sweep with the VE-FD in FP32; "dfd": its DFD-only
+        # form; otherwise the FP64 bound sweep.
+        dfd, ve = s.Bound_call_gpu(0.0, run_vefd=what != "dfd", fp32=what == "mixed")
         out = {"profile": what, "points": points, "dfd": dfd, "vefd": ve}
         if what == "decision":
             out["radius"] = dfd * (1 - 0.5e-9)
@@ -373,9 +375,10 @@ def main() -> int:
                     help="skip the mixed-precision run reported beside the FP64 line")
     ap.add_argument("--batch-pairs", type=int, default=4096,
                     help="pairs of the batched workload reported beside the main line (0: skip)")
-    ap.add_argument("--profile", choices=["decision", "bounds", "copy", "learned"],
+    ap.add_argument("--profile", choices=["decision", "bounds", "mixed", "dfd", "copy", "learned"],
                     help="run one launch of a kernel for rocprofv3 and exit: the FP64 decision at "
-                         "DFD (1 - 0.5e-9) or the fused bound sweep on the pair of --points, or a "
+                         "DFD (1 - 0.5e-9) or the fused bound sweep on the pair of --points (FP64, "
+                         "VE-FD in FP32, or DFD alone), or a "
                          "256 MiB device-to-device copy (byte-counter calibration), or one "
                          "learned-bracket run of the 4,096-pair batch")
     args = ap.parse_args()
