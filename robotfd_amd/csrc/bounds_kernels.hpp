@@ -389,6 +389,11 @@ __device__ inline void bounds_chain(const BoundsArgs &a, const int band, BoundsS
         dbot[t] = big;
         vbot[t] = -big;  // column 0's weights then set the row's VE-FD entry value
     }
+    // Row 0 of the pair: its VE-FD values are +big at every column (the
+    // weights' boundary rule).  Entering it at +big, with +big from row -1,
+    // gives exactly that whatever its weights, so the producers apply no rule
+    // to it.
+    if (band == 0 && lane == 0) vbot[0] = big;
     double dout[C], vout[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) dout[c] = vout[c] = big;
@@ -1087,14 +1092,14 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         if (edge) {
             // Boundary and padding cells overwrite their VE-FD weights.  A real
             // branch (the asm keeps the compiler from turning it into selects
-            // on every step): it is taken while the band fills, and in the
-            // first and last bands.
+            // on every step): it is taken while the band fills and at the last
+            // band's closing steps.
             asm volatile("");
 #pragma unroll
             for (int k = 0; k < RP; ++k) {
                 const int t = tfirst + k;
                 const long long i = i0 + t;
-                const bool pad = i >= rows;  // padding rows of the last band pass VE-FD values up unchanged
+                const bool pad = i >= rows;  // padding rows of the last band pass VE-FD values up unchanged (at its closing steps)
                 if (j < 0) {
                     // Before the lane's first column: weights that leave the
                     // chain's row state at its initial values (DFD +big,
@@ -1148,14 +1153,27 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         if (lane == 0) lds_put(&prod[pw], s + 1);
         slot = (slot + 1 == KS) ? 0 : slot + 1;
     };
-    const bool edge_band = band == 0 || band + 1 == a.nbands;
+    // Steps that apply the boundary rules: the first 64 (lanes before or at
+    // their first column) and the last steps of the last band (padding rows).
+    // Row 0's rule needs no weights after column 0 (the chain enters the row
+    // at +big).  A padding row only has to pass
+    // the terminal row's VE-FD value R up through the terminal lane at the
+    // band's last step: its right weight -big there, and its own value +big
+    // from the step before, which its top weight +big one step earlier gives.
+    // Nothing else reads padding rows (the terminal DFD value is its row's,
+    // lanes above the terminal lane feed only each other, and the last band
+    // has no successor), so the last band runs its other steps without the
+    // rules.  Run with them on every step, the first band set the pace of the
+    // bands above it and the last band fell behind the band below: 109.5
+    // against 100.5 ms at 200k x 200k.
+    const int tail = band + 1 == a.nbands ? nsteps - 2 : nsteps;
     if constexpr (C == 2) {
         // The carried values alternate between the A and B registers from
         // one column to the next instead of being copied.
         for (int s = 0; s < nsteps; ++s) {
             step_begin(s, 0);
             double *w = wring + slot * (C * R * mode_weights(MODE) * 64) + lane;
-            const bool edge = s < 64 || edge_band;
+            const bool edge = s < 64 || s >= tail;
             const int j = C * (s - lane);
             column(j, cA, cB, w, edge);
             column(j + 1, cB, cA, w + R * mode_weights(MODE) * 64, edge);
@@ -1178,10 +1196,8 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
         // per lane, as a ring of 4 does with pairs) and 16% longer (4 rows).
         static_assert(KS >= 3, "step pairs need a ring of three steps");
         constexpr bool QUADS = KS >= 12 && NPW >= 5;
-        // The boundary rules apply in the first 64 steps (lanes before or at
-        // their first column) and in the first and last bands (row 0, padding
-        // rows): those steps run a copy of the loop that checks them, the
-        // others one without the check.
+        // Steps with the boundary rules (above) run a copy of the loop that
+        // checks them, the others one without the check.
         auto one = [&](auto edge_c, const int s, const Carry &cp_, Carry &cn_, const bool first, const bool last,
                        const int ahead) __attribute__((always_inline)) {
             if (first) step_begin(s, last ? 0 : ahead);
@@ -1212,10 +1228,15 @@ __device__ inline void bounds_producer(const BoundsArgs &a, const int band, cons
                 ++s;
             }
         };
-        // 64 is a multiple of the trip length, so the second loop starts a
-        // trip at a chunk boundary, as the first does.
-        steps_to(std::true_type{}, edge_band ? nsteps : (nsteps < 64 ? nsteps : 64));
-        steps_to(std::false_type{}, nsteps);
+        // Chunk stores are checked at a trip's first step, so every trip
+        // starts at a multiple of its length: 64 is one, and the last band's
+        // closing steps start at a multiple of four.
+        const int edge_end = nsteps < 64 ? nsteps : 64;
+        int tail_start = tail < nsteps ? (tail & ~3) : nsteps;
+        tail_start = tail_start > edge_end ? tail_start : edge_end;
+        steps_to(std::true_type{}, edge_end);
+        steps_to(std::false_type{}, tail_start);
+        steps_to(std::true_type{}, nsteps);
     }
 }
 
