@@ -35,12 +35,12 @@ KERNEL_SYMBOL = {k: f"rfd::band_sweep<{op}, 3, {t}, 1, {_R}, false>" for k, op, 
                  (("vefd_f64", 1, "double"), ("vefd_f32", 1, "float"), ("decide_f64", 2, "double"),
                   ("decide_f32", 2, "float"), ("dfd", 0, "double"))}
 # Fused DFD + VE-FD sweep: bounds_sweep<dims, rows per lane, producer waves,
-# columns per step>; at 200k rows the library picks 13 rows per lane, 10
-# producer waves and one column per step.
+# columns per step, mode (3: FP64)>; at 200k rows the library picks 13 rows
+# per lane, 10 producer waves and one column per step.
 _BCFG = int(os.environ.get("RFD_BOUNDS", "1310"))
 _BC, _BBASE = (2, _BCFG - 20000) if _BCFG >= 20000 else (1, _BCFG)
 _BR, _BN = (_BBASE // 100, _BBASE % 100) if _BBASE >= 1000 else (_BBASE // 10, _BBASE % 10)
-KERNEL_SYMBOL["bounds_f64"] = f"rfd::bounds_sweep<3, {_BR}, {_BN}, {_BC}>"
+KERNEL_SYMBOL["bounds_f64"] = f"rfd::bounds_sweep<3, {_BR}, {_BN}, {_BC}, 3>"
 
 
 def flops_per_launch(kind: str, dims: int, m: int, n: int, evaluated_per_launch: float) -> float:
