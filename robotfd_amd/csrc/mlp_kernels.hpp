@@ -4,11 +4,14 @@
 //   hidden layers: a = relu(x W + b) stored in BF16; head: p = sigmoid(a w + b) in FP32.
 //
 // The hidden layers' forward and backward products run through one GEMM
-// kernel on v_mfma_f32_16x16x16_bf16 (64x64 output tile per workgroup of four
-// waves, each wave a 32x32 block of 2x2 MFMA tiles).  The three operand
-// layouts training needs (X W, X^T dZ, dZ W^T) are strides of the same kernel.
-// Epilogues fuse bias + ReLU (forward) and the ReLU mask of the layer below
-// (backward).  Master weights and Adam moments are FP32.
+// kernel on gfx950's v_mfma_f32_16x16x32_bf16: a 64x64 output tile per
+// workgroup of four waves, each wave a 32x32 block of 2x2 MFMA tiles, with
+// the operands of each 32-deep K step staged through LDS (each element read
+// from memory once per workgroup, each wave's fragments one 16-byte LDS read
+// apiece).  The three operand layouts training needs (X W, X^T dZ, dZ W^T)
+// are strides of the same kernel.  Epilogues fuse bias + ReLU (forward) and
+// the ReLU mask of the layer below (backward).  Master weights and Adam
+// moments are FP32.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -18,8 +21,9 @@
 
 namespace rfd {
 
-typedef short mfma_s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 mfma_bf8 __attribute__((ext_vector_type(8)));
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+typedef unsigned int mfma_u4 __attribute__((ext_vector_type(4)));
 
 enum GemmEpi { EPI_F32 = 0, EPI_BIAS_RELU_BF16 = 1, EPI_MASK_BF16 = 2 };
 
@@ -34,45 +38,111 @@ struct GemmArgs {
     const uint16_t *mask;  // EPI_MASK_BF16: keep where mask[i*N + j] > 0
 };
 
-// MFMA 16x16x16 BF16 operand layout: lane l supplies A[l%16][4*(l/16) + t]
-// and B[4*(l/16) + t][l%16], t = 0..3; it receives D[4*(l/16) + t][l%16].
-template <int EPI>
+// K step of the staged tiles: A as [64 rows][k], B transposed as [64
+// columns][k], k contiguous, rows 16 bytes longer than the staged K (row
+// starts 16-byte aligned).  KSTEPS = 1 streams K through one step's tiles;
+// KSTEPS > 1 stages the whole K (at most 32 KSTEPS) at once, so that every
+// load of the launch is issued before the first multiply: the layers' K is
+// 400 or 256, where a streamed launch waits on memory once per step.
+constexpr int kGemmK = 32;
+constexpr int kGemmStepsAll = 13;  // the network's widest K, 400, in 32-deep steps
+
+// MFMA 16x16x32 BF16 operand layout: lane l supplies A[l%16][8*(l/16) + t]
+// and B[8*(l/16) + t][l%16], t = 0..7; it receives D[4*(l/16) + t][l%16],
+// t = 0..3.
+template <int EPI, int KSTEPS>
 __global__ __launch_bounds__(256) void mlp_gemm(GemmArgs g) {
-    const int lane = (int)(threadIdx.x & 63);
-    const int wave = (int)(threadIdx.x >> 6);
-    const int row0 = (int)blockIdx.y * 64 + (wave >> 1) * 32;
-    const int col0 = (int)blockIdx.x * 64 + (wave & 1) * 32;
-    const int r16 = lane & 15, kq = (lane >> 4) * 4;
+    constexpr int kGemmLd = KSTEPS * kGemmK + 8;
+    __shared__ __attribute__((aligned(16))) uint16_t As[64 * kGemmLd];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[64 * kGemmLd];
+    const int tid = (int)threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int tr0 = (int)blockIdx.y * 64, tc0 = (int)blockIdx.x * 64;
+    const int row0 = tr0 + (wave >> 1) * 32;
+    const int col0 = tc0 + (wave & 1) * 32;
+    const int r16 = lane & 15, kq = (lane >> 4) * 4, k8 = (lane >> 4) * 8;
+    // Staging: thread t loads tile row (and tile column) t / 4, k t % 4 * 8 .. + 7.
+    const int lr = tid >> 2, lk = (tid & 3) * 8;
+    const int li = tr0 + lr, lj = tc0 + lr;
+    // One thread's eight A and eight B elements of the K step at k0, packed
+    // two to a word; zero outside the matrices.  A row whose k runs
+    // contiguously (sak = 1, the forward's X) with 16-byte aligned rows is
+    // one 16-byte load.
+    const bool avec = g.sak == 1 && (g.sai & 7) == 0 && ((uintptr_t)g.A & 15) == 0;
+    auto load = [&](const int k0, mfma_u4 &va, mfma_u4 &vb) __attribute__((always_inline)) {
+        const int kb = k0 + lk;
+        unsigned a2[4] = {0, 0, 0, 0}, b2[4] = {0, 0, 0, 0};
+        if (avec && kb + 8 <= g.K && li < g.M) {
+            const mfma_u4 v = *(const mfma_u4 *)(g.A + (long long)li * g.sai + kb);
+            a2[0] = v[0], a2[1] = v[1], a2[2] = v[2], a2[3] = v[3];
+        } else if (li < g.M) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = kb + u;
+                const unsigned e = k < g.K ? g.A[(long long)li * g.sai + (long long)k * g.sak] : 0u;
+                a2[u >> 1] |= e << (16 * (u & 1));
+            }
+        }
+        if (lj < g.N) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = kb + u;
+                const unsigned e = k < g.K ? g.B[(long long)k * g.sbk + (long long)lj * g.sbj] : 0u;
+                b2[u >> 1] |= e << (16 * (u & 1));
+            }
+        }
+        va = mfma_u4{a2[0], a2[1], a2[2], a2[3]};
+        vb = mfma_u4{b2[0], b2[1], b2[2], b2[3]};
+    };
     mfma_f4 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = mfma_f4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < g.K; k0 += 16) {
-        mfma_s4 fa[2], fb[2];
+    // The fragments of the K step staged at column kk of the tiles.
+    auto multiply = [&](const int kk) __attribute__((always_inline)) {
+        mfma_bf8 fa[2], fb[2];
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-            const int i = row0 + a * 16 + r16;
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int k = k0 + kq + t;
-                fa[a][t] = (i < g.M && k < g.K) ? (short)g.A[(long long)i * g.sai + (long long)k * g.sak] : (short)0;
-            }
+            const mfma_u4 v = *(const mfma_u4 *)&As[((wave >> 1) * 32 + a * 16 + r16) * kGemmLd + kk + k8];
+            fa[a] = __builtin_bit_cast(mfma_bf8, v);
         }
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            const int j = col0 + b * 16 + r16;
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int k = k0 + kq + t;
-                fb[b][t] = (j < g.N && k < g.K) ? (short)g.B[(long long)k * g.sbk + (long long)j * g.sbj] : (short)0;
-            }
+            const mfma_u4 v = *(const mfma_u4 *)&Bs[((wave & 1) * 32 + b * 16 + r16) * kGemmLd + kk + k8];
+            fb[b] = __builtin_bit_cast(mfma_bf8, v);
         }
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int b = 0; b < 2; ++b)
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa[a], fb[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    };
+    if constexpr (KSTEPS > 1) {
+        // The whole K at once (the host launches this form for K <= 32 KSTEPS).
+        mfma_u4 va[KSTEPS], vb[KSTEPS];
+#pragma unroll
+        for (int st = 0; st < KSTEPS; ++st) load(st * kGemmK, va[st], vb[st]);
+#pragma unroll
+        for (int st = 0; st < KSTEPS; ++st) {
+            *(mfma_u4 *)&As[lr * kGemmLd + st * kGemmK + lk] = va[st];
+            *(mfma_u4 *)&Bs[lr * kGemmLd + st * kGemmK + lk] = vb[st];
+        }
+        __syncthreads();
+        for (int k0 = 0; k0 < g.K; k0 += kGemmK) multiply(k0);
+    } else {
+        mfma_u4 va, vb;
+        load(0, va, vb);
+        for (int k0 = 0; k0 < g.K; k0 += kGemmK) {
+            __syncthreads();  // the previous step's fragments are read
+            *(mfma_u4 *)&As[lr * kGemmLd + lk] = va;
+            *(mfma_u4 *)&Bs[lr * kGemmLd + lk] = vb;
+            __syncthreads();
+            // The next step's elements are in flight while this one's multiply.
+            if (k0 + kGemmK < g.K) load(k0 + kGemmK, va, vb);
+            multiply(0);
+        }
     }
 #pragma unroll
     for (int a = 0; a < 2; ++a)

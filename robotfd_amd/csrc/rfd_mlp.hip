@@ -39,10 +39,16 @@ GemmArgs gemm_args(const uint16_t *A, long long sai, long long sak, const uint16
     return g;
 }
 
+// The layers' products (K = 400 or 256) stage their whole K at once; the
+// weight gradients' (K = the batch's rows) stream it.
 template <int EPI>
 void gemm(hipStream_t st, const GemmArgs &g) {
     dim3 grid((unsigned)((g.N + 63) / 64), (unsigned)((g.M + 63) / 64));
-    hipLaunchKernelGGL((mlp_gemm<EPI>), grid, dim3(256), 0, st, g);
+    if (g.K <= kGemmStepsAll * kGemmK) {
+        hipLaunchKernelGGL((mlp_gemm<EPI, kGemmStepsAll>), grid, dim3(256), 0, st, g);
+    } else {
+        hipLaunchKernelGGL((mlp_gemm<EPI, 1>), grid, dim3(256), 0, st, g);
+    }
 }
 
 // Backward + gradient buffer fill from the activations of the last forward.
